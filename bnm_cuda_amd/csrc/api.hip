@@ -1,0 +1,666 @@
+// api.hip -- the C ABI of include/bnm_hip.h: config, index tables, noise, and
+// the persistent context that runs the whole path on one GPU.
+//
+// Host-side pieces restate the reference driver: env defaults (constants.h:93-116,
+// helpers.cpp:4-20), output_ts / pair / window tables (run_CMAES.cpp:476-594),
+// the noise array (run_CMAES.cpp:640-657).  All compute runs on the device.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/bnm_hip.h"
+#include "kernels.hpp"
+
+using namespace bnm;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                        \
+    do {                                                                                    \
+        hipError_t _e = (expr);                                                             \
+        if (_e != hipSuccess)                                                               \
+            return fail(BNM_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));      \
+    } while (0)
+
+// get_env_or_default of helpers.cpp:4-20: the int overload parses with atoi,
+// the double overload with atof and returns float.
+int env_int(const char* k, int dflt) {
+    const char* v = std::getenv(k);
+    return v ? std::atoi(v) : dflt;
+}
+float env_float(const char* k, double dflt) {
+    const char* v = std::getenv(k);
+    return v ? (float)std::atof(v) : (float)dflt;
+}
+
+template <class T>
+int dalloc(T** p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+    if (e != hipSuccess) return fail(BNM_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return BNM_OK;
+}
+
+}  // namespace
+
+struct bnm_ctx {
+    bnm_config cfg{};
+    int N = 0, B = 0, ots = 0, np = 0, nw = 0, nseg = 0, m = 0;
+    long long nwp = 0;
+    bool f64 = true;      // integration precision
+    bool fc64 = true;     // Gram precision
+    hipStream_t st = nullptr;
+    double* SC64 = nullptr;
+    float* SC32 = nullptr;
+    float* noise = nullptr;
+    long long noise_cap = 0;
+    double *G = nullptr, *wEE = nullptr, *wEI = nullptr, *wIE = nullptr, *wIEfic = nullptr,
+           *wIEscratch = nullptr;
+    void* bold = nullptr;
+    double* ext = nullptr;
+    int *failed = nullptr, *trials = nullptr, *unstable = nullptr;
+    int *seg_start = nullptr, *seg_len = nullptr, *pidx = nullptr;
+    double *mean = nullptr, *inv = nullptr, *wmean = nullptr, *winv = nullptr;
+    void *fc = nullptr, *wfc = nullptr, *fcd = nullptr;
+    double *emp_c = nullptr, *ks_u = nullptr, *ks_fe = nullptr;
+    double emp_mean = 0, emp_ssd = 0;
+    int* bins = nullptr;
+    int* nanf = nullptr;
+    double *corr = nullptr, *pen = nullptr;
+    hipEvent_t ev[6] = {};
+    double timings[6] = {0, 0, 0, 0, 0, 0};
+    std::string kname = "none";
+    bool have_sc = false, have_noise = false, have_emp = false, have_params = false;
+    int fast_ls = 0, fast_ks = 0;
+};
+
+extern "C" {
+
+const char* bnm_last_error(void) { return g_err.c_str(); }
+
+int bnm_default_config(bnm_config* c) {
+    if (!c) return fail(BNM_E_ARG, "null config");
+    std::memset(c, 0, sizeof(*c));
+    c->nodes = 0;
+    c->n_sims = 0;
+    c->time_steps = 0;
+    c->bold_tr = 1000;
+    c->n_vols_remove = -1;
+    c->window_size = 30;
+    c->window_step = 5;
+    c->exc_interhemispheric = env_int("BNM_EXC_INTERHEM", 1) != 0;
+    c->do_fic = 1;
+    c->adjust_fic = env_int("BNM_NUMERICAL_FIC", 1) != 0;
+    c->max_fic_trials = env_int("BNM_MAX_FIC_TRIALS_CMAES", 10);
+    c->i_sampling_start = env_int("BNM_I_SAMPLING_START", 1000);
+    c->i_sampling_end = env_int("BNM_I_SAMPLING_END", 10000);
+    c->init_delta = (double)env_float("BNM_INIT_DELTA", 0.02);
+    c->noise_period_ms = 1000;
+    c->calculate_fic_penalty = 1;
+    c->sim_only = 0;
+    c->no_fcd = 0;
+    c->use_fc_diff = env_int("BNM_USE_FC_DIFF", 1) != 0;
+    c->max_cost = (double)env_int("BNM_MAX_COST", 2);
+    c->fic_penalty_scale = (double)env_float("BNM_CMAES_FIC_PENALTY_SCALE", 2.0);
+    c->fic_reject_failed = env_int("BNM_CMAES_FIC_REJECT_FAILED", 0) != 0;
+    c->analytical_fic = 1;
+    c->precision = BNM_PREC_F32;
+    c->fc_precision = BNM_FC_F32;
+    c->kernel = BNM_KERNEL_AUTO;
+    c->sims_per_block = 0;
+    return BNM_OK;
+}
+
+int bnm_output_ts(int time_steps, int bold_tr) {
+    if (bold_tr <= 0 || time_steps < 0) return fail(BNM_E_ARG, "bad time_steps / bold_tr");
+    // run_CMAES.cpp:476-477 evaluates int(T / (TR_s / 0.001)) + 1 in floating
+    // point; the kernels sample every ts with ts % TR == 0, i.e. T/TR + 1 times.
+    const int n = time_steps / bold_tr + 1;
+    const double TR = (double)bold_tr / 1000;
+    const int ref = (int)(time_steps / (TR / 0.001)) + 1;
+    if (ref != n) return fail(BNM_E_UNSUP, "output_ts of the reference formula differs from T/TR+1");
+    return n;
+}
+
+int bnm_n_pairs(int nodes, int exc) {
+    if (nodes < 2) return fail(BNM_E_ARG, "nodes < 2");
+    if (exc && (nodes % 2)) return fail(BNM_E_ARG, "exc_interhemispheric needs an even node count");
+    int n = nodes * (nodes - 1) / 2;
+    if (exc) n -= (nodes / 2) * (nodes / 2);
+    return n;
+}
+
+int bnm_finalize_config(bnm_config* c) {
+    if (!c) return fail(BNM_E_ARG, "null config");
+    if (c->nodes < 2 || c->nodes > 1024) return fail(BNM_E_ARG, "nodes must be in [2, 1024]");
+    if (c->n_sims < 1) return fail(BNM_E_ARG, "n_sims < 1");
+    if (c->bold_tr <= 0 || c->time_steps < 0) return fail(BNM_E_ARG, "bad time_steps / bold_tr");
+    if (c->n_vols_remove < 0) c->n_vols_remove = env_int("BNM_BOLD_REMOVE", 30) * 1000 / c->bold_tr;
+    if (c->noise_period_ms < 1) return fail(BNM_E_ARG, "noise_period_ms < 1");
+    if (c->window_step < 1 || c->window_size < 1) return fail(BNM_E_ARG, "bad FCD window");
+    if (c->i_sampling_end < c->i_sampling_start) return fail(BNM_E_ARG, "bad FIC sampling window");
+    const int ots = bnm_output_ts(c->time_steps, c->bold_tr);
+    if (ots < 0) return ots;
+    if (!c->sim_only) {
+        if (ots - c->n_vols_remove < 2) return fail(BNM_E_ARG, "fewer than 2 volumes left for FC");
+        const int np = bnm_n_pairs(c->nodes, c->exc_interhemispheric);
+        if (np < 0) return np;
+        if (!c->no_fcd && bnm_n_windows(c) < 2) return fail(BNM_E_ARG, "fewer than 2 FCD windows");
+    }
+    return BNM_OK;
+}
+
+int bnm_window_bounds(const bnm_config* c, int* starts, int* ends) {
+    const int ots = c->time_steps / c->bold_tr + 1;
+    const int corr_len = ots - c->n_vols_remove;
+    const int first = c->window_size / 2 + c->n_vols_remove;
+    const int last = corr_len - 1 - c->window_size / 2 + c->n_vols_remove;
+    int n = 0;
+    for (int ctr = first; ctr <= last; ctr += c->window_step) {
+        int s = ctr - c->window_size / 2, e = ctr + c->window_size / 2;
+        if (s < 0) s = 0;
+        if (e >= ots) e = ots - 1;
+        if (starts) { starts[n] = s; ends[n] = e; }
+        ++n;
+    }
+    return n;
+}
+
+int bnm_n_windows(const bnm_config* c) {
+    if (!c || c->bold_tr <= 0 || c->window_step < 1) return fail(BNM_E_ARG, "bad config");
+    return bnm_window_bounds(c, nullptr, nullptr);
+}
+
+int bnm_make_noise(unsigned seed, long long count, float* out) {
+    if (!out || count < 0) return fail(BNM_E_ARG, "bad noise buffer");
+    std::mt19937 gen(seed);
+    std::normal_distribution<float> nd(0, 1);
+    for (long long i = 0; i < count; ++i) out[i] = nd(gen);
+    return BNM_OK;
+}
+
+int bnm_make_noise_chunked(unsigned seed, long long count, float* out, int threads) {
+    if (!out || count < 0) return fail(BNM_E_ARG, "bad noise buffer");
+    constexpr long long kChunk = 1LL << 20;
+    const long long nchunks = (count + kChunk - 1) / kChunk;
+    int nt = threads > 0 ? threads : (int)std::thread::hardware_concurrency();
+    if (nt < 1) nt = 1;
+    if (nt > nchunks) nt = (int)(nchunks > 0 ? nchunks : 1);
+    auto work = [&](int t) {
+        for (long long c = t; c < nchunks; c += nt) {
+            std::seed_seq sq{seed, (unsigned)(c & 0xffffffffu), (unsigned)(c >> 32)};
+            std::mt19937 gen(sq);
+            std::normal_distribution<float> nd(0, 1);
+            const long long b = c * kChunk, e = std::min(count, b + kChunk);
+            for (long long i = b; i < e; ++i) out[i] = nd(gen);
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) pool.emplace_back(work, t);
+    work(0);
+    for (auto& th : pool) th.join();
+    return BNM_OK;
+}
+
+void bnm_ctx_destroy(bnm_ctx* x) {
+    if (!x) return;
+    void* bufs[] = {x->SC64, x->SC32, x->noise, x->G, x->wEE, x->wEI, x->wIE, x->wIEfic,
+                    x->wIEscratch, x->bold, x->ext, x->failed, x->trials, x->unstable,
+                    x->seg_start, x->seg_len, x->pidx, x->mean, x->inv, x->wmean, x->winv, x->fc,
+                    x->wfc, x->fcd, x->emp_c, x->ks_u, x->ks_fe, x->bins, x->nanf, x->corr, x->pen};
+    for (void* p : bufs)
+        if (p) (void)hipFree(p);
+    for (auto& e : x->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (x->st) (void)hipStreamDestroy(x->st);
+    delete x;
+}
+
+static int ctx_alloc(bnm_ctx* x) {
+    const bnm_config& c = x->cfg;
+    const size_t N = x->N, B = x->B;
+    const size_t rsz = x->f64 ? 8 : 4, fsz = x->fc64 ? 8 : 4;
+    int r;
+    if ((r = dalloc(&x->SC64, N * N))) return r;
+    if ((r = dalloc(&x->SC32, N * N))) return r;
+    if ((r = dalloc(&x->G, B))) return r;
+    if ((r = dalloc(&x->wEE, B * N))) return r;
+    if ((r = dalloc(&x->wEI, B * N))) return r;
+    if ((r = dalloc(&x->wIE, B * N))) return r;
+    if ((r = dalloc(&x->wIEfic, B * N))) return r;
+    if ((r = dalloc(&x->wIEscratch, B * N))) return r;
+    HIPCHK(hipMalloc(&x->bold, B * x->ots * N * rsz));
+    if ((r = dalloc(&x->ext, B * 6 * N))) return r;
+    if ((r = dalloc(&x->failed, B))) return r;
+    if ((r = dalloc(&x->trials, B))) return r;
+    if ((r = dalloc(&x->unstable, B))) return r;
+    if ((r = dalloc(&x->corr, B))) return r;
+    if ((r = dalloc(&x->pen, B))) return r;
+    if (!c.sim_only) {
+        std::vector<int> st(x->nseg), ln(x->nseg);
+        st[0] = c.n_vols_remove;
+        ln[0] = x->ots - c.n_vols_remove;
+        std::vector<int> ws(x->nw > 0 ? x->nw : 1), we(x->nw > 0 ? x->nw : 1);
+        bnm_window_bounds(&c, ws.data(), we.data());
+        for (int w = 0; w + 1 < x->nseg; ++w) {
+            st[w + 1] = ws[w];
+            ln[w + 1] = we[w] - ws[w] + 1;
+        }
+        if ((r = dalloc(&x->seg_start, x->nseg))) return r;
+        if ((r = dalloc(&x->seg_len, x->nseg))) return r;
+        HIPCHK(hipMemcpy(x->seg_start, st.data(), x->nseg * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(x->seg_len, ln.data(), x->nseg * sizeof(int), hipMemcpyHostToDevice));
+        // pair table (run_CMAES.cpp:494-524): pidx[i*N+j] = position of pair (i,j), i > j
+        std::vector<int> pidx(N * N, -1);
+        const int rh = x->N / 2;
+        int p = 0;
+        for (int i = 0; i < x->N; ++i)
+            for (int j = 0; j < i; ++j) {
+                if (c.exc_interhemispheric && ((i < rh) ^ (j < rh))) continue;
+                pidx[(size_t)i * N + j] = p++;
+            }
+        if ((r = dalloc(&x->pidx, N * N))) return r;
+        HIPCHK(hipMemcpy(x->pidx, pidx.data(), N * N * sizeof(int), hipMemcpyHostToDevice));
+        if ((r = dalloc(&x->mean, B * x->nseg * N))) return r;
+        if ((r = dalloc(&x->inv, B * x->nseg * N))) return r;
+        HIPCHK(hipMalloc(&x->fc, B * x->np * fsz));
+        if ((r = dalloc(&x->emp_c, x->np))) return r;
+        if (!c.no_fcd) {
+            HIPCHK(hipMalloc(&x->wfc, B * (size_t)x->nw * x->np * fsz));
+            if ((r = dalloc(&x->wmean, B * x->nw))) return r;
+            if ((r = dalloc(&x->winv, B * x->nw))) return r;
+            HIPCHK(hipMalloc(&x->fcd, B * (size_t)x->nwp * fsz));
+            if ((r = dalloc(&x->nanf, B))) return r;
+        }
+    }
+    for (auto& e : x->ev) HIPCHK(hipEventCreate(&e));
+    return BNM_OK;
+}
+
+bnm_ctx* bnm_ctx_create(const bnm_config* cfg, int* status) {
+    auto set = [&](int s) { if (status) *status = s; };
+    if (!cfg) { set(fail(BNM_E_ARG, "null config")); return nullptr; }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        set(fail(BNM_E_NODEV, "no HIP device"));
+        return nullptr;
+    }
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess ||
+        std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        set(fail(BNM_E_NODEV, "the current HIP device is not gfx950"));
+        return nullptr;
+    }
+    bnm_config c = *cfg;
+    int r = bnm_finalize_config(&c);
+    if (r) { set(r); return nullptr; }
+    if (c.precision == BNM_PREC_F64 && (c.kernel == BNM_KERNEL_MFMA)) {
+        set(fail(BNM_E_UNSUP, "the MFMA integration kernel runs in FP32/BF16 only"));
+        return nullptr;
+    }
+    if (c.precision == BNM_PREC_BF16 && c.kernel != BNM_KERNEL_MFMA && c.kernel != BNM_KERNEL_AUTO) {
+        set(fail(BNM_E_UNSUP, "BF16 coupling needs the MFMA kernel"));
+        return nullptr;
+    }
+    auto* x = new bnm_ctx;
+    x->cfg = c;
+    x->N = c.nodes;
+    x->B = c.n_sims;
+    x->ots = c.time_steps / c.bold_tr + 1;
+    x->f64 = c.precision == BNM_PREC_F64;
+    x->fc64 = c.fc_precision == BNM_FC_F64;
+    if (!c.sim_only) {
+        x->np = bnm_n_pairs(c.nodes, c.exc_interhemispheric);
+        x->nw = c.no_fcd ? 0 : bnm_n_windows(&c);
+        x->nwp = (long long)x->nw * (x->nw - 1) / 2;
+        x->nseg = 1 + x->nw;
+    }
+    x->fast_ls = env_int("BNM_FAST_LS", 0);
+    x->fast_ks = env_int("BNM_FAST_KS", 0);
+    if (hipStreamCreateWithFlags(&x->st, hipStreamNonBlocking) != hipSuccess) {
+        set(fail(BNM_E_HIP, "hipStreamCreate failed"));
+        delete x;
+        return nullptr;
+    }
+    r = ctx_alloc(x);
+    if (r) {
+        bnm_ctx_destroy(x);
+        set(r);
+        return nullptr;
+    }
+    set(BNM_OK);
+    return x;
+}
+
+int bnm_ctx_set_sc(bnm_ctx* x, const double* SC) {
+    if (!x || !SC) return fail(BNM_E_ARG, "null argument");
+    const size_t n = (size_t)x->N * x->N;
+    std::vector<float> f(n);
+    for (size_t i = 0; i < n; ++i) f[i] = (float)SC[i];
+    HIPCHK(hipMemcpy(x->SC64, SC, n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(x->SC32, f.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    x->have_sc = true;
+    return BNM_OK;
+}
+
+int bnm_ctx_set_noise(bnm_ctx* x, const float* noise, long long len) {
+    if (!x || !noise) return fail(BNM_E_ARG, "null argument");
+    const long long need = x->cfg.noise_period_ms * 10LL * x->N * 2;
+    if (len < need) return fail(BNM_E_ARG, "noise array shorter than noise_period_ms*10*N*2");
+    if (x->noise_cap < need) {
+        if (x->noise) (void)hipFree(x->noise);
+        x->noise = nullptr;
+        HIPCHK(hipMalloc(&x->noise, need * sizeof(float)));
+        x->noise_cap = need;
+    }
+    HIPCHK(hipMemcpy(x->noise, noise, need * sizeof(float), hipMemcpyHostToDevice));
+    x->have_noise = true;
+    return BNM_OK;
+}
+
+int bnm_ctx_set_empirical(bnm_ctx* x, const double* emp_fc, int n_fc, const double* emp_fcd,
+                          int n_fcd) {
+    if (!x) return fail(BNM_E_ARG, "null context");
+    if (x->cfg.sim_only) { x->have_emp = true; return BNM_OK; }
+    if (!emp_fc || n_fc != x->np) return fail(BNM_E_SIZE, "empirical FC size does not match the pair count");
+    // centred empirical FC and its norm (the constant half of the Pearson)
+    long double s = 0;
+    for (int i = 0; i < n_fc; ++i) s += emp_fc[i];
+    const double mean = (double)(s / n_fc);
+    std::vector<double> c(n_fc);
+    long double ss = 0;
+    for (int i = 0; i < n_fc; ++i) {
+        c[i] = emp_fc[i] - mean;
+        ss += (long double)c[i] * c[i];
+    }
+    x->emp_mean = mean;
+    x->emp_ssd = std::sqrt((double)ss);
+    HIPCHK(hipMemcpy(x->emp_c, c.data(), n_fc * sizeof(double), hipMemcpyHostToDevice));
+    if (!x->cfg.no_fcd) {
+        if (!emp_fcd || n_fcd < 1) return fail(BNM_E_SIZE, "empty empirical FCD");
+        std::vector<double> u(emp_fcd, emp_fcd + n_fcd);
+        std::sort(u.begin(), u.end());
+        std::vector<double> uu, fe;
+        for (int i = 0; i < n_fcd; ++i) {
+            if (i + 1 < n_fcd && u[i + 1] == u[i]) continue;  // last of a run of ties
+            uu.push_back(u[i]);
+            fe.push_back((double)(i + 1) / n_fcd);
+        }
+        if (x->ks_u) { (void)hipFree(x->ks_u); x->ks_u = nullptr; }
+        if (x->ks_fe) { (void)hipFree(x->ks_fe); x->ks_fe = nullptr; }
+        if (x->bins) { (void)hipFree(x->bins); x->bins = nullptr; }
+        x->m = (int)uu.size();
+        int r;
+        if ((r = dalloc(&x->ks_u, uu.size()))) return r;
+        if ((r = dalloc(&x->ks_fe, fe.size()))) return r;
+        if ((r = dalloc(&x->bins, (size_t)x->B * (2 * x->m + 1)))) return r;
+        HIPCHK(hipMemcpy(x->ks_u, uu.data(), uu.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(x->ks_fe, fe.data(), fe.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    x->have_emp = true;
+    return BNM_OK;
+}
+
+int bnm_ctx_set_params(bnm_ctx* x, const double* G, const double* wEE, const double* wEI,
+                       const double* wIE, const double* wIE_fic) {
+    if (!x || !G || !wEE || !wEI) return fail(BNM_E_ARG, "null argument");
+    const size_t B = x->B, N = x->N;
+    HIPCHK(hipMemcpy(x->G, G, B * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(x->wEE, wEE, B * N * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(x->wEI, wEI, B * N * sizeof(double), hipMemcpyHostToDevice));
+    if (wIE) HIPCHK(hipMemcpy(x->wIE, wIE, B * N * sizeof(double), hipMemcpyHostToDevice));
+    else HIPCHK(hipMemset(x->wIE, 0, B * N * sizeof(double)));
+    if (x->cfg.do_fic && !x->cfg.analytical_fic) {
+        if (!wIE_fic) return fail(BNM_E_ARG, "w_IE_fic is required when analytical_fic == 0");
+        HIPCHK(hipMemcpy(x->wIEfic, wIE_fic, B * N * sizeof(double), hipMemcpyHostToDevice));
+    }
+    x->have_params = true;
+    return BNM_OK;
+}
+
+static int run_integration(bnm_ctx* x, SimArgs& a) {
+    const bnm_config& c = x->cfg;
+    int kern = c.kernel;
+    if (kern == BNM_KERNEL_AUTO) kern = BNM_KERNEL_FAST;
+    char name[128];
+    if (kern == BNM_KERNEL_PLAIN) {
+        if (x->f64) HIPCHK(launch_sim_plain<double>(x->SC64, a, x->st));
+        else HIPCHK(launch_sim_plain<float>(x->SC32, a, x->st));
+        std::snprintf(name, sizeof name, "plain_%s(threads=%d)", x->f64 ? "f64" : "f32",
+                      ((x->N + 63) / 64) * 64);
+    } else if (kern == BNM_KERNEL_FAST) {
+        const size_t el = x->f64 ? 8 : 4;
+        int ls = x->fast_ls > 0 ? x->fast_ls : 4, ks = x->fast_ks > 0 ? x->fast_ks : 1;
+        FastShape s = fast_shape(x->N, ls, ks, el);
+        // otherwise the first register-resident shape in order of preference
+        static const int pref[][2] = {{4, 1}, {2, 1}, {1, 1}, {2, 2}, {1, 2}, {4, 2},
+                                      {1, 4}, {2, 4}, {1, 8}, {2, 8}, {1, 16}};
+        for (const auto& p : pref) {
+            if (s.ok) break;
+            s = fast_shape(x->N, p[0], p[1], el);
+            if (s.ok) { ls = p[0]; ks = p[1]; }
+        }
+        if (!s.ok) return fail(BNM_E_UNSUP, "no register-resident shape fits this node count");
+        if (x->f64) HIPCHK(launch_sim_fast<double>(x->SC64, a, ls, ks, x->st));
+        else HIPCHK(launch_sim_fast<float>(x->SC32, a, ls, ks, x->st));
+        std::snprintf(name, sizeof name, "split_%s(LS=%d,KS=%d,NW=%d,KC=%d,SS=%d,threads=%d)",
+                      x->f64 ? "f64" : "f32", s.LS, s.KS, s.NW, s.KC, s.SS, s.threads);
+    } else {
+        return fail(BNM_E_UNSUP, "integration kernel not available");
+    }
+    x->kname = name;
+    return BNM_OK;
+}
+
+int bnm_ctx_run(bnm_ctx* x) {
+    if (!x) return fail(BNM_E_ARG, "null context");
+    if (!x->have_sc || !x->have_noise || !x->have_params || (!x->cfg.sim_only && !x->have_emp))
+        return fail(BNM_E_ARG, "SC, noise, parameters and the empirical target must be set first");
+    const bnm_config& c = x->cfg;
+    hipStream_t st = x->st;
+    HIPCHK(hipEventRecord(x->ev[0], st));
+    if (c.do_fic && c.analytical_fic)
+        HIPCHK(launch_fic_analytical(x->SC64, x->N, x->B, x->G, x->wEE, x->wEI, x->wIEfic,
+                                     x->unstable, st));
+    HIPCHK(hipEventRecord(x->ev[1], st));
+    SimArgs a{};
+    a.N = x->N;
+    a.B = x->B;
+    a.time_steps = c.time_steps;
+    a.bold_tr = c.bold_tr;
+    a.output_ts = x->ots;
+    a.n_vols_remove = c.n_vols_remove;
+    a.adjust_fic = c.do_fic && c.adjust_fic;
+    a.max_fic_trials = c.max_fic_trials;
+    a.i_start = c.i_sampling_start;
+    a.i_end = c.i_sampling_end;
+    a.init_delta = c.init_delta;
+    a.noise_period = c.noise_period_ms;
+    a.noise = x->noise;
+    a.G = x->G;
+    a.wEE = x->wEE;
+    a.wEI = x->wEI;
+    a.wIE_in = c.do_fic ? x->wIEfic : x->wIE;
+    a.wIE_out = c.do_fic ? x->wIEfic : x->wIEscratch;
+    a.bold = x->bold;
+    a.ext = x->ext;
+    a.fic_failed = x->failed;
+    a.fic_trials = x->trials;
+    int r = run_integration(x, a);
+    if (r) return r;
+    HIPCHK(hipEventRecord(x->ev[2], st));
+    if (!c.sim_only) {
+        if (x->f64) {
+            HIPCHK(launch_seg_moments<double>((const double*)x->bold, x->B, x->N, x->ots, x->nseg,
+                                              x->seg_start, x->seg_len, x->mean, x->inv, st));
+            if (x->fc64)
+                HIPCHK((launch_gram_fc<double, double, double>(
+                    (const double*)x->bold, x->B, x->N, x->ots, x->nseg, x->seg_start, x->seg_len,
+                    x->mean, x->inv, x->pidx, x->np, (double*)x->fc, (double*)x->wfc, st)));
+            else
+                HIPCHK((launch_gram_fc<double, float, float>(
+                    (const double*)x->bold, x->B, x->N, x->ots, x->nseg, x->seg_start, x->seg_len,
+                    x->mean, x->inv, x->pidx, x->np, (float*)x->fc, (float*)x->wfc, st)));
+        } else {
+            HIPCHK(launch_seg_moments<float>((const float*)x->bold, x->B, x->N, x->ots, x->nseg,
+                                             x->seg_start, x->seg_len, x->mean, x->inv, st));
+            if (x->fc64)
+                HIPCHK((launch_gram_fc<float, double, double>(
+                    (const float*)x->bold, x->B, x->N, x->ots, x->nseg, x->seg_start, x->seg_len,
+                    x->mean, x->inv, x->pidx, x->np, (double*)x->fc, (double*)x->wfc, st)));
+            else
+                HIPCHK((launch_gram_fc<float, float, float>(
+                    (const float*)x->bold, x->B, x->N, x->ots, x->nseg, x->seg_start, x->seg_len,
+                    x->mean, x->inv, x->pidx, x->np, (float*)x->fc, (float*)x->wfc, st)));
+        }
+    }
+    HIPCHK(hipEventRecord(x->ev[3], st));
+    GofArgs g{};
+    g.B = x->B;
+    g.N = x->N;
+    g.np = x->np;
+    g.nwp = x->nwp;
+    g.m = x->m;
+    g.emp_fc_c = x->emp_c;
+    g.emp_fc_ssd = x->emp_ssd;
+    g.emp_fc_mean = x->emp_mean;
+    g.ks_u = x->ks_u;
+    g.ks_fe = x->ks_fe;
+    g.bins = x->bins;
+    g.nan_flag = x->nanf;
+    g.ext = x->ext;
+    g.fic_failed = x->failed;
+    g.sim_only = c.sim_only;
+    g.no_fcd = c.no_fcd;
+    g.use_fc_diff = c.use_fc_diff;
+    g.calc_penalty = c.calculate_fic_penalty;
+    g.fic_reject = c.fic_reject_failed;
+    g.max_cost = c.max_cost;
+    g.fic_scale = c.fic_penalty_scale;
+    g.corr = x->corr;
+    g.fic_pen = x->pen;
+    if (!c.sim_only && !c.no_fcd) {
+        if (x->fc64) {
+            HIPCHK(launch_row_moments<double>((const double*)x->wfc, (long long)x->B * x->nw, x->np,
+                                              x->wmean, x->winv, st));
+            HIPCHK((launch_gram_fcd<double, double, double>((const double*)x->wfc, x->B, x->nw, x->np,
+                                                            x->wmean, x->winv, (double*)x->fcd, st)));
+        } else {
+            HIPCHK(launch_row_moments<float>((const float*)x->wfc, (long long)x->B * x->nw, x->np,
+                                             x->wmean, x->winv, st));
+            HIPCHK((launch_gram_fcd<float, float, float>((const float*)x->wfc, x->B, x->nw, x->np,
+                                                         x->wmean, x->winv, (float*)x->fcd, st)));
+        }
+    }
+    HIPCHK(hipEventRecord(x->ev[4], st));
+    if (!c.sim_only && !c.no_fcd) {
+        HIPCHK(launch_fill_i32(x->bins, (long long)x->B * (2 * x->m + 1), 0, st));
+        HIPCHK(launch_fill_i32(x->nanf, x->B, 0, st));
+        if (x->fc64) HIPCHK(launch_ks_bin<double>((const double*)x->fcd, g, st));
+        else HIPCHK(launch_ks_bin<float>((const float*)x->fcd, g, st));
+    }
+    if (x->fc64) HIPCHK(launch_gof<double>((const double*)x->fc, g, st));
+    else HIPCHK(launch_gof<float>((const float*)x->fc, g, st));
+    HIPCHK(hipEventRecord(x->ev[5], st));
+    HIPCHK(hipStreamSynchronize(st));
+    float ms = 0;
+    for (int i = 0; i < 4; ++i) {
+        HIPCHK(hipEventElapsedTime(&ms, x->ev[i], x->ev[i + 1]));
+        x->timings[i] = ms;
+    }
+    HIPCHK(hipEventElapsedTime(&ms, x->ev[4], x->ev[5]));
+    x->timings[4] = ms;
+    HIPCHK(hipEventElapsedTime(&ms, x->ev[0], x->ev[5]));
+    x->timings[5] = ms;
+    return BNM_OK;
+}
+
+int bnm_ctx_get_outputs(bnm_ctx* x, double* corr, double* pen, double* wIE_fic, double* BOLD,
+                        double* S_E, double* I_E, double* r_E, double* S_I, double* I_I,
+                        double* r_I, int* failed, int* trials, double* fc, double* fcd) {
+    if (!x) return fail(BNM_E_ARG, "null context");
+    const size_t B = x->B, N = x->N;
+    if (corr) HIPCHK(hipMemcpy(corr, x->corr, B * sizeof(double), hipMemcpyDeviceToHost));
+    if (pen) HIPCHK(hipMemcpy(pen, x->pen, B * sizeof(double), hipMemcpyDeviceToHost));
+    if (wIE_fic && x->cfg.do_fic)
+        HIPCHK(hipMemcpy(wIE_fic, x->wIEfic, B * N * sizeof(double), hipMemcpyDeviceToHost));
+    if (failed) HIPCHK(hipMemcpy(failed, x->failed, B * sizeof(int), hipMemcpyDeviceToHost));
+    if (trials) HIPCHK(hipMemcpy(trials, x->trials, B * sizeof(int), hipMemcpyDeviceToHost));
+    double* ext_out[6] = {S_E, I_E, r_E, S_I, I_I, r_I};
+    bool any_ext = false;
+    for (double* p : ext_out) any_ext = any_ext || p;
+    if (any_ext) {
+        std::vector<double> e(B * 6 * N);
+        HIPCHK(hipMemcpy(e.data(), x->ext, e.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int i = 0; i < 6; ++i)
+            if (ext_out[i])
+                for (size_t b = 0; b < B; ++b)
+                    std::memcpy(ext_out[i] + b * N, e.data() + (b * 6 + i) * N, N * sizeof(double));
+    }
+    auto fetch = [&](double* dst, const void* src, size_t n, bool is64) -> int {
+        if (is64) {
+            HIPCHK(hipMemcpy(dst, src, n * sizeof(double), hipMemcpyDeviceToHost));
+        } else {
+            std::vector<float> t(n);
+            HIPCHK(hipMemcpy(t.data(), src, n * sizeof(float), hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < n; ++i) dst[i] = t[i];
+        }
+        return BNM_OK;
+    };
+    int r;
+    if (BOLD && (r = fetch(BOLD, x->bold, B * x->ots * N, x->f64))) return r;
+    if (!x->cfg.sim_only) {
+        if (fc && (r = fetch(fc, x->fc, B * x->np, x->fc64))) return r;
+        if (fcd && !x->cfg.no_fcd && (r = fetch(fcd, x->fcd, B * x->nwp, x->fc64))) return r;
+    }
+    return BNM_OK;
+}
+
+int bnm_ctx_timings(bnm_ctx* x, double* ms, int n) {
+    if (!x || !ms) return fail(BNM_E_ARG, "null argument");
+    const int k = n < 6 ? n : 6;
+    for (int i = 0; i < k; ++i) ms[i] = x->timings[i];
+    return k;
+}
+
+const char* bnm_ctx_kernel_name(bnm_ctx* x) { return x ? x->kname.c_str() : ""; }
+
+int bnm_run_simulations(const bnm_config* cfg, double* corr, double* fic_penalties,
+                        const double* G_list, const double* w_EE_list, const double* w_EI_list,
+                        const double* w_IE_list, double* w_IE_fic, const double* SC,
+                        const float* noise, long long noise_len, const double* emp_FC_tril,
+                        int n_emp_fc, const double* emp_FCD_tril, int n_emp_fcd, double* BOLD,
+                        double* S_E, double* I_E, double* r_E, double* S_I, double* I_I,
+                        double* r_I, int* FIC_failed, int* fic_n_trials, double* fc_trils,
+                        double* fcd_trils) {
+    int st = BNM_OK;
+    bnm_ctx* x = bnm_ctx_create(cfg, &st);
+    if (!x) return st;
+    int r = bnm_ctx_set_sc(x, SC);
+    if (!r) r = bnm_ctx_set_noise(x, noise, noise_len);
+    if (!r) r = bnm_ctx_set_empirical(x, emp_FC_tril, n_emp_fc, emp_FCD_tril, n_emp_fcd);
+    if (!r) r = bnm_ctx_set_params(x, G_list, w_EE_list, w_EI_list, w_IE_list, w_IE_fic);
+    if (!r) r = bnm_ctx_run(x);
+    if (!r)
+        r = bnm_ctx_get_outputs(x, corr, fic_penalties, w_IE_fic, BOLD, S_E, I_E, r_E, S_I, I_I,
+                                r_I, FIC_failed, fic_n_trials, fc_trils, fcd_trils);
+    bnm_ctx_destroy(x);
+    return r;
+}
+
+}  // extern "C"

@@ -1,0 +1,307 @@
+// fc_kernels.hip -- FC, windowed FC and FCD as Gram products of z-scored
+// columns on MFMA (bold_stats / window_bold_stats / fc / window_fc_stats /
+// fcd of bnm.cu:317-551; fc_tril / fcd_tril of check_fit.cpp:19-136).
+//
+// Pearson r(x_i, x_j) = sum_t z_ti z_tj with z_ti = (x_ti - mean_i) / ssd_i and
+// ssd_i = sqrt(sum_t (x_ti - mean_i)^2).  The moments are FP64 sums with
+// Neumaier compensation; the cross products run on MFMA in the Gram precision
+// (FP64: v_mfma_f64_16x16x4f64, FP32: v_mfma_f32_16x16x4f32).
+#include "kernels.hpp"
+
+namespace bnm {
+
+namespace {
+
+// Neumaier-compensated running sum
+struct KSum {
+    double s = 0, c = 0;
+    __device__ __forceinline__ void add(double x) {
+        const double t = s + x;
+        c += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
+        s = t;
+    }
+    __device__ __forceinline__ double value() const { return s + c; }
+};
+
+template <class T> __device__ __forceinline__ double to_d(T v) { return (double)v; }
+
+// block-wide sum of a double (blockDim.x multiple of 64, <= 1024)
+__device__ double block_sum(double v, double* red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) red[w] = v;
+    __syncthreads();
+    double t = 0;
+    const int nwv = blockDim.x >> 6;
+    for (int i = 0; i < nwv; ++i) t += red[i];
+    return t;
+}
+
+}  // namespace
+
+// ---- node moments per (sim, segment) -----------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void k_seg_moments(const T* __restrict__ bold, int N,
+                                                     int output_ts, int n_seg,
+                                                     const int* __restrict__ seg_start,
+                                                     const int* __restrict__ seg_len,
+                                                     double* __restrict__ mean,
+                                                     double* __restrict__ inv_ssd) {
+    const int seg = blockIdx.x, sim = blockIdx.y;
+    const int t0 = seg_start[seg], len = seg_len[seg];
+    const T* x = bold + ((size_t)sim * output_ts + t0) * N;
+    for (int j = threadIdx.x; j < N; j += blockDim.x) {
+        KSum s;
+        for (int t = 0; t < len; ++t) s.add(to_d(x[(size_t)t * N + j]));
+        const double m = s.value() / len;
+        KSum q;
+        for (int t = 0; t < len; ++t) {
+            const double d = to_d(x[(size_t)t * N + j]) - m;
+            q.add(d * d);
+        }
+        const size_t o = ((size_t)sim * n_seg + seg) * N + j;
+        mean[o] = m;
+        inv_ssd[o] = 1.0 / sqrt(q.value());
+    }
+}
+
+// ---- row moments (window-FC rows over the edge axis) -------------------------
+template <class T>
+__global__ __launch_bounds__(256) void k_row_moments(const T* __restrict__ x, int len,
+                                                     double* __restrict__ mean,
+                                                     double* __restrict__ inv_ssd) {
+    __shared__ double red[16];
+    const size_t row = blockIdx.x;
+    const T* r = x + row * len;
+    KSum s;
+    for (int i = threadIdx.x; i < len; i += blockDim.x) s.add(to_d(r[i]));
+    const double tot = block_sum(s.s, red) + block_sum(s.c, red);
+    const double m = tot / len;
+    KSum q;
+    for (int i = threadIdx.x; i < len; i += blockDim.x) {
+        const double d = to_d(r[i]) - m;
+        q.add(d * d);
+    }
+    const double ss = block_sum(q.s, red) + block_sum(q.c, red);
+    if (threadIdx.x == 0) {
+        mean[row] = m;
+        inv_ssd[row] = 1.0 / sqrt(ss);
+    }
+}
+
+// ---- MFMA Gram ------------------------------------------------------------------
+namespace {
+
+template <class Tc> struct MF;
+// Output layout of the 16x16 result: element r of lane l sits at
+// (row(l, r), l % 16).  v_mfma_f32_16x16x4f32 stacks each lane's 4 values
+// down consecutive rows; v_mfma_f64_16x16x4f64 interleaves them by 4 rows.
+template <> struct MF<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+    static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct MF<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ int row(int lane, int r) { return 4 * r + (lane >> 4); }
+    static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+
+constexpr int TM = 64;    // output tile edge (4 waves x 32x32)
+constexpr int TK = 16;    // K chunk staged in LDS
+constexpr int LDT = TM + 16;
+
+struct GramItem {
+    const void* base;     // element (k, m) at base[k*ldk + m*ldm]
+    int K, M, ldk, ldm;
+    const double* mu;     // [M]
+    const double* is;     // [M]
+    void* out;            // pair-indexed output of this item
+};
+
+__device__ __forceinline__ void tile_pair(int p, int& ti, int& tj) {
+    int t = (int)((sqrt(8.0 * p + 1.0) - 1.0) * 0.5);
+    while ((t + 1) * (t + 2) / 2 <= p) ++t;
+    while (t * (t + 1) / 2 > p) --t;
+    ti = t;
+    tj = p - t * (t + 1) / 2;
+}
+
+// One 64x64 lower-triangle output tile (ti >= tj) of the Gram of item `it`.
+// pidx: pair index table [M*M] (-1 = not in the list) or nullptr for the
+// closed form i*(i-1)/2 + j.
+template <class Tin, class Tc, class Tout>
+__device__ void gram_tile(const GramItem& it, int ti, int tj, const int* __restrict__ pidx,
+                          Tc (*As)[LDT], Tc (*Bs)[LDT]) {
+    using V4 = typename MF<Tc>::v4;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wi = w >> 1, wj = w & 1;
+    const Tin* X = reinterpret_cast<const Tin*>(it.base);
+    V4 acc[2][2];
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) acc[a][b] = V4{0, 0, 0, 0};
+    const bool mfast = it.ldm == 1;
+    for (int k0 = 0; k0 < it.K; k0 += TK) {
+#pragma unroll
+        for (int r = 0; r < (TK * TM) / 256; ++r) {
+            const int e = tid + 256 * r;
+            const int m = mfast ? (e % TM) : (e / TK);
+            const int k = mfast ? (e / TM) : (e % TK);
+            const int kg = k0 + k;
+            const int mi = ti * TM + m, mj = tj * TM + m;
+            Tc za = 0, zb = 0;
+            if (kg < it.K) {
+                if (mi < it.M)
+                    za = (Tc)((to_d(X[(size_t)kg * it.ldk + (size_t)mi * it.ldm]) - it.mu[mi]) * it.is[mi]);
+                if (mj < it.M)
+                    zb = (Tc)((to_d(X[(size_t)kg * it.ldk + (size_t)mj * it.ldm]) - it.mu[mj]) * it.is[mj]);
+            }
+            As[k][m] = za;
+            Bs[k][m] = zb;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < TK / 4; ++kk) {
+            const int kr = kk * 4 + (lane >> 4);
+            Tc a[2], b[2];
+            for (int q = 0; q < 2; ++q) {
+                a[q] = As[kr][wi * 32 + q * 16 + (lane & 15)];
+                b[q] = Bs[kr][wj * 32 + q * 16 + (lane & 15)];
+            }
+            for (int qa = 0; qa < 2; ++qa)
+                for (int qb = 0; qb < 2; ++qb) acc[qa][qb] = MF<Tc>::mma(a[qa], b[qb], acc[qa][qb]);
+        }
+        __syncthreads();
+    }
+    Tout* out = reinterpret_cast<Tout*>(it.out);
+    for (int qa = 0; qa < 2; ++qa)
+        for (int qb = 0; qb < 2; ++qb)
+            for (int r = 0; r < 4; ++r) {
+                const int row = ti * TM + wi * 32 + qa * 16 + MF<Tc>::row(lane, r);
+                const int col = tj * TM + wj * 32 + qb * 16 + (lane & 15);
+                if (row < it.M && col < row) {
+                    const long long p = pidx ? pidx[(size_t)row * it.M + col]
+                                             : (long long)row * (row - 1) / 2 + col;
+                    if (p >= 0) out[p] = (Tout)acc[qa][qb][r];
+                }
+            }
+}
+
+}  // namespace
+
+template <class Tin, class Tc, class Tout>
+__global__ __launch_bounds__(256) void k_gram_fc(const Tin* __restrict__ bold, int N, int output_ts,
+                                                 int n_seg, const int* __restrict__ seg_start,
+                                                 const int* __restrict__ seg_len,
+                                                 const double* __restrict__ mean,
+                                                 const double* __restrict__ inv_ssd,
+                                                 const int* __restrict__ pidx, int np,
+                                                 Tout* __restrict__ fc, Tout* __restrict__ wfc) {
+    __shared__ Tc As[TK][LDT];
+    __shared__ Tc Bs[TK][LDT];
+    int ti, tj;
+    tile_pair(blockIdx.x, ti, tj);
+    const int seg = blockIdx.y, sim = blockIdx.z;
+    GramItem it;
+    it.base = bold + ((size_t)sim * output_ts + seg_start[seg]) * N;
+    it.K = seg_len[seg];
+    it.M = N;
+    it.ldk = N;
+    it.ldm = 1;
+    const size_t mo = ((size_t)sim * n_seg + seg) * N;
+    it.mu = mean + mo;
+    it.is = inv_ssd + mo;
+    it.out = seg == 0 ? (void*)(fc + (size_t)sim * np)
+                      : (void*)(wfc + ((size_t)sim * (n_seg - 1) + (seg - 1)) * np);
+    gram_tile<Tin, Tc, Tout>(it, ti, tj, pidx, As, Bs);
+}
+
+template <class Tin, class Tc, class Tout>
+__global__ __launch_bounds__(256) void k_gram_fcd(const Tin* __restrict__ wfc, int nw, int np,
+                                                  const double* __restrict__ mean,
+                                                  const double* __restrict__ inv_ssd,
+                                                  Tout* __restrict__ fcd) {
+    __shared__ Tc As[TK][LDT];
+    __shared__ Tc Bs[TK][LDT];
+    int ti, tj;
+    tile_pair(blockIdx.x, ti, tj);
+    const int sim = blockIdx.z;
+    GramItem it;
+    it.base = wfc + (size_t)sim * nw * np;
+    it.K = np;
+    it.M = nw;
+    it.ldk = 1;
+    it.ldm = np;
+    it.mu = mean + (size_t)sim * nw;
+    it.is = inv_ssd + (size_t)sim * nw;
+    it.out = fcd + (size_t)sim * ((long long)nw * (nw - 1) / 2);
+    gram_tile<Tin, Tc, Tout>(it, ti, tj, nullptr, As, Bs);
+}
+
+// ---- launchers ---------------------------------------------------------------
+template <class T>
+hipError_t launch_seg_moments(const T* bold, int B, int N, int output_ts, int n_seg,
+                              const int* seg_start, const int* seg_len, double* mean,
+                              double* inv_ssd, hipStream_t st) {
+    const int threads = N >= 256 ? 256 : ((N + 63) / 64) * 64;
+    hipLaunchKernelGGL(k_seg_moments<T>, dim3(n_seg, B), dim3(threads), 0, st, bold, N, output_ts,
+                       n_seg, seg_start, seg_len, mean, inv_ssd);
+    return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_row_moments(const T* x, long long rows, int len, double* mean, double* inv_ssd,
+                              hipStream_t st) {
+    hipLaunchKernelGGL(k_row_moments<T>, dim3((unsigned)rows), dim3(256), 0, st, x, len, mean, inv_ssd);
+    return hipGetLastError();
+}
+
+template <class Tin, class Tc, class Tout>
+hipError_t launch_gram_fc(const Tin* bold, int B, int N, int output_ts, int n_seg,
+                          const int* seg_start, const int* seg_len, const double* mean,
+                          const double* inv_ssd, const int* pidx, int np, Tout* fc, Tout* wfc,
+                          hipStream_t st) {
+    const int nt = (N + TM - 1) / TM;
+    hipLaunchKernelGGL((k_gram_fc<Tin, Tc, Tout>), dim3(nt * (nt + 1) / 2, n_seg, B), dim3(256), 0,
+                       st, bold, N, output_ts, n_seg, seg_start, seg_len, mean, inv_ssd, pidx, np,
+                       fc, wfc);
+    return hipGetLastError();
+}
+
+template <class Tin, class Tc, class Tout>
+hipError_t launch_gram_fcd(const Tin* wfc, int B, int nw, int np, const double* mean,
+                           const double* inv_ssd, Tout* fcd, hipStream_t st) {
+    const int nt = (nw + TM - 1) / TM;
+    hipLaunchKernelGGL((k_gram_fcd<Tin, Tc, Tout>), dim3(nt * (nt + 1) / 2, 1, B), dim3(256), 0, st,
+                       wfc, nw, np, mean, inv_ssd, fcd);
+    return hipGetLastError();
+}
+
+#define BNM_INST_MOM(T)                                                                           \
+    template hipError_t launch_seg_moments<T>(const T*, int, int, int, int, const int*, const int*, \
+                                              double*, double*, hipStream_t);                     \
+    template hipError_t launch_row_moments<T>(const T*, long long, int, double*, double*, hipStream_t);
+BNM_INST_MOM(float)
+BNM_INST_MOM(double)
+
+#define BNM_INST_FC(Tin, Tc)                                                                      \
+    template hipError_t launch_gram_fc<Tin, Tc, Tc>(const Tin*, int, int, int, int, const int*,   \
+                                                    const int*, const double*, const double*,     \
+                                                    const int*, int, Tc*, Tc*, hipStream_t);
+BNM_INST_FC(double, double)
+BNM_INST_FC(double, float)
+BNM_INST_FC(float, double)
+BNM_INST_FC(float, float)
+
+#define BNM_INST_FCD(Tc)                                                                          \
+    template hipError_t launch_gram_fcd<Tc, Tc, Tc>(const Tc*, int, int, int, const double*,      \
+                                                    const double*, Tc*, hipStream_t);
+BNM_INST_FCD(double)
+BNM_INST_FCD(float)
+
+}  // namespace bnm

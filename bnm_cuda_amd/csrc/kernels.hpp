@@ -1,0 +1,93 @@
+// kernels.hpp -- host-side launch functions of the HIP kernels (internal to
+// libbnm_hip.so; the public surface is include/bnm_hip.h).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include "model.hpp"
+
+namespace bnm {
+
+// ---- integration (sim_kernels.hip) ----------------------------------------
+struct FastShape {
+    int LS;        // lanes per node within a wave (1, 2, 4)
+    int KS;        // wave groups splitting the coupling dot
+    int NW;        // waves per wave group
+    int kc;        // SC values per lane
+    int KC;        // register array size (template)
+    int SS;        // LDS slice stride (elements)
+    int threads;   // workgroup size
+    size_t smem;   // LDS bytes
+    bool ok;
+};
+FastShape fast_shape(int N, int LS, int KS, size_t elem);
+
+template <class R>
+hipError_t launch_sim_plain(const R* SC, const SimArgs& a, hipStream_t st);
+template <class R>
+hipError_t launch_sim_fast(const R* SC, const SimArgs& a, int LS, int KS, hipStream_t st);
+
+// MFMA-batched integration (sim_mfma.hip): b simulations per workgroup.
+struct MfmaShape {
+    int b;         // simulations per workgroup
+    int threads;
+    size_t smem;
+    bool ok;
+};
+MfmaShape mfma_shape(int N, int b, bool bf16);
+hipError_t launch_sim_mfma(const float* SC, const SimArgs& a, int b, bool bf16, hipStream_t st);
+
+// analytical FIC on the device (fic_kernel.hip), fic.cpp:130-207
+hipError_t launch_fic_analytical(const double* SC, int N, int B, const double* G,
+                                 const double* wEE, const double* wEI, double* wIE_out,
+                                 int* unstable, hipStream_t st);
+
+// ---- FC / FCD (fc_kernels.hip) ----------------------------------------------
+// Per (sim, segment) node moments of BOLD over [seg_start, seg_start+seg_len):
+// mean and 1/sqrt(sum (x-mean)^2), FP64 compensated sums.
+template <class T>
+hipError_t launch_seg_moments(const T* bold, int B, int N, int output_ts, int n_seg,
+                              const int* seg_start, const int* seg_len, double* mean,
+                              double* inv_ssd, hipStream_t st);
+// Gram of z-scored columns.  Mode FC: items (sim, seg), X[t][n] =
+// bold[(start+t)*N + n]; outputs the pair list (pair table pidx[N*N], -1 =
+// skipped) into fc (seg 0) and wfc (seg >= 1).  Mode FCD: items sim, X[e][w]
+// = wfc[(sim*nw + w)*np + e]; outputs the wi>wj triangle into fcd.
+template <class Tin, class Tc, class Tout>
+hipError_t launch_gram_fc(const Tin* bold, int B, int N, int output_ts, int n_seg,
+                          const int* seg_start, const int* seg_len, const double* mean,
+                          const double* inv_ssd, const int* pidx, int np, Tout* fc, Tout* wfc,
+                          hipStream_t st);
+template <class Tin, class Tc, class Tout>
+hipError_t launch_gram_fcd(const Tin* wfc, int B, int nw, int np, const double* mean,
+                           const double* inv_ssd, Tout* fcd, hipStream_t st);
+// Row moments of the window-FC matrix (mean, 1/ssd over the np edges).
+template <class T>
+hipError_t launch_row_moments(const T* x, long long rows, int len, double* mean, double* inv_ssd,
+                              hipStream_t st);
+
+// ---- fit (gof_kernels.hip) ---------------------------------------------------
+struct GofArgs {
+    int B, N, np;
+    long long nwp;               // FCD values per sim
+    int m;                       // unique empirical FCD breakpoints
+    const double* emp_fc_c;      // [np] empirical FC minus its mean
+    double emp_fc_ssd;           // sqrt(sum emp_fc_c^2)
+    double emp_fc_mean;
+    const double* ks_u;          // [m] sorted unique empirical FCD values
+    const double* ks_fe;         // [m] empirical CDF at ks_u
+    int* bins;                   // [B][2m+1] scratch, zero before launch
+    int* nan_flag;               // [B] scratch, zero before launch
+    const double* ext;           // [B][6][N] (r_E = row 2)
+    const int* fic_failed;       // [B]
+    int sim_only, no_fcd, use_fc_diff, calc_penalty, fic_reject;
+    double max_cost, fic_scale;
+    double* corr;                // [B]
+    double* fic_pen;             // [B]
+};
+template <class T>
+hipError_t launch_ks_bin(const T* fcd, const GofArgs& g, hipStream_t st);
+template <class T>
+hipError_t launch_gof(const T* fc, const GofArgs& g, hipStream_t st);
+hipError_t launch_fill_i32(int* p, long long n, int v, hipStream_t st);
+
+}  // namespace bnm

@@ -1,0 +1,211 @@
+// model.hpp -- reduced Wong-Wang (Deco 2014) + Balloon-Windkessel constants and
+// per-node update steps, shared by every integration kernel.
+//
+// Constants follow constants.h:27-90 of the reference.  Two numeric flavours:
+//  * Exact<double>: the reference CPU expression order (bnm.cpp:181-211) with
+//    libm-equivalent exp/pow, used by the FP64 kernels so that their BOLD can be
+//    compared against the FP64 CPU oracle at 1e-6 relative.
+//  * Fast<float>: hardware exp2/log2/rcp (v_exp_f32, v_log_f32, v_rcp_f32) with
+//    a series branch for the rate function's removable singularity.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdint>
+
+namespace bnm {
+
+// constants.h:27-90
+struct C64 {
+    static constexpr double dt = 0.1;
+    static constexpr double model_dt = 0.001;
+    static constexpr double sqrt_dt = 0.316227;       // truncated, as in constants.h:29
+    static constexpr double J_NMDA = 0.15;
+    static constexpr double a_E = 310, b_E = 125, d_E = 0.16;
+    static constexpr double a_I = 615, b_I = 177, d_I = 0.087;
+    static constexpr double b_a_ratio_E = 125.0 / 310.0;
+    static constexpr double gamma_E = 0.641 / 1000.0;
+    static constexpr double gamma_I = 1.0 / 1000.0;
+    static constexpr double tau_E = 100, tau_I = 10;
+    static constexpr double itau_E = 1.0 / 100, itau_I = 1.0 / 10;
+    static constexpr double sigma = 0.01;
+    static constexpr double wE_I0 = 1.0 * 0.382;
+    static constexpr double wI_I0 = 0.7 * 0.382;
+    static constexpr double rho = 0.34, alpha = 0.32, tau = 0.98;
+    static constexpr double y = 1.0 / 0.41, kappa = 1.0 / 0.65;
+    static constexpr double V0 = 0.02 * 100;
+    static constexpr double ialpha = 1.0 / 0.32, itau = 1.0 / 0.98, oneminrho = 1.0 - 0.34;
+    static constexpr double k1 = 3.72, k2 = 0.527, k3 = 0.53;
+    // FIC steady state of the isolated node (constants.h:57-68)
+    static constexpr double tau_I_s = 0.01, gamma_I_s = 1.0;
+    static constexpr double I_E_ss = 0.3773805650, S_E_ss = 0.1647572075;
+};
+
+// log2(e) and log2(1 - rho) for the exp2/log2 forms
+constexpr double kLog2e = 1.4426950408889634;
+constexpr double kLog2OneMinRho = -0.5994620704162712;  // log2(0.66)
+
+// Per-node Euler-Maruyama substep.  In: coupling c = sum_k SC[j,k] S_E[k]
+// (previous substep), noise draws xiE/xiI.  Updates S_E/S_I, returns I_E, I_I,
+// r_E, r_I of this substep (the values the reference records at TR samples).
+template <class R> struct Node;
+
+// FP64, reference expression order of bnm.cpp:181-200 (IEEE ops; the kernels
+// that use it compile with contraction off).
+template <> struct Node<double> {
+    static __device__ __forceinline__ void step(double c, double G, double wEE, double wEI,
+                                                double wIE, double xiE, double xiI, double& SE,
+                                                double& SI, double& IE, double& II, double& rE,
+                                                double& rI) {
+#pragma clang fp contract(off)
+        IE = C64::wE_I0 + wEE * SE + c * G * C64::J_NMDA - wIE * SI;
+        II = C64::wI_I0 + wEI * SE - SI;
+        const double eE = exp(-C64::d_E * (C64::a_E * IE - C64::b_E));
+        const double eI = exp(-C64::d_I * (C64::a_I * II - C64::b_I));
+        rE = (C64::a_E * IE - C64::b_E) / (1 - eE);
+        rI = (C64::a_I * II - C64::b_I) / (1 - eI);
+        const double dSE = xiE * C64::sigma * C64::sqrt_dt +
+                           C64::dt * ((1 - SE) * C64::gamma_E * rE - (SE / C64::tau_E));
+        const double dSI = xiI * C64::sigma * C64::sqrt_dt +
+                           C64::dt * (C64::gamma_I * rI - (SI / C64::tau_I));
+        SE = fmax(0.0, fmin(SE + dSE, 1.0));
+        SI = fmax(0.0, fmin(SI + dSI, 1.0));
+    }
+};
+
+// x / (1 - exp(-d x)), FP32.  Near x = 0 the quotient is 1/d + x/2 + d x^2/12
+// (series of the removable singularity); the series is used for |x| < 0.1,
+// where the direct form would lose digits to cancellation.
+template <int which>
+__device__ __forceinline__ float rate_f32(float x) {
+    constexpr float nd_log2e = (float)(-(which == 0 ? C64::d_E : C64::d_I) * kLog2e);
+    constexpr float inv_d = (float)(1.0 / (which == 0 ? C64::d_E : C64::d_I));
+    constexpr float d12 = (float)((which == 0 ? C64::d_E : C64::d_I) / 12.0);
+    const float e = __builtin_amdgcn_exp2f(nd_log2e * x);
+    const float direct = x * __builtin_amdgcn_rcpf(1.0f - e);
+    const float series = inv_d + x * (0.5f + d12 * x);
+    return fabsf(x) < 0.1f ? series : direct;
+}
+
+template <> struct Node<float> {
+    static __device__ __forceinline__ void step(float c, float GJ, float wEE, float wEI, float wIE,
+                                                float xiE, float xiI, float& SE, float& SI,
+                                                float& IE, float& II, float& rE, float& rI) {
+        constexpr float nz = (float)(C64::sigma * C64::sqrt_dt);
+        constexpr float dt_gE = (float)(C64::dt * C64::gamma_E);
+        constexpr float dt_itE = (float)(C64::dt / C64::tau_E);
+        constexpr float dt_gI = (float)(C64::dt * C64::gamma_I);
+        constexpr float dt_itI = (float)(C64::dt / C64::tau_I);
+        IE = (float)C64::wE_I0 + wEE * SE + c * GJ - wIE * SI;
+        II = (float)C64::wI_I0 + wEI * SE - SI;
+        rE = rate_f32<0>((float)C64::a_E * IE - (float)C64::b_E);
+        rI = rate_f32<1>((float)C64::a_I * II - (float)C64::b_I);
+        const float dSE = xiE * nz + (dt_gE * rE - dt_gE * rE * SE - dt_itE * SE);
+        const float dSI = xiI * nz + (dt_gI * rI - dt_itI * SI);
+        SE = __builtin_amdgcn_fmed3f(SE + dSE, 0.0f, 1.0f);
+        SI = __builtin_amdgcn_fmed3f(SI + dSI, 0.0f, 1.0f);
+    }
+};
+
+// Balloon-Windkessel state; one Euler step of 1 ms (bnm.cpp:205-211), update
+// order: x from old f, f from new x, nu and q from old f and (q) new nu.
+template <class R> struct BW;
+
+template <> struct BW<double> {
+    double x = 0.0, f = 1.0, nu = 1.0, q = 1.0;
+    __device__ __forceinline__ void reset() { x = 0.0; f = 1.0; nu = 1.0; q = 1.0; }
+    __device__ __forceinline__ void step(double SE) {
+#pragma clang fp contract(off)
+        x = x + C64::model_dt * (SE - C64::kappa * x - C64::y * (f - 1.0));
+        const double f_new = f + C64::model_dt * x;
+        nu = nu + C64::model_dt * C64::itau * (f - pow(nu, C64::ialpha));
+        q = q + C64::model_dt * C64::itau *
+                    (f * (1.0 - pow(C64::oneminrho, (1.0 / f))) / C64::rho -
+                     pow(nu, C64::ialpha) * q / nu);
+        f = f_new;
+    }
+    __device__ __forceinline__ double bold() const {
+#pragma clang fp contract(off)
+        return C64::V0 * (C64::k1 * (1 - q) + C64::k2 * (1 - q / nu) + C64::k3 * (1 - nu));
+    }
+};
+
+// FP32 flavour: the BW state stays in FP64 and only the 1 ms increments are
+// formed in FP32 (v_exp_f32 / v_log_f32 / v_rcp_f32).  f, nu and q sit near
+// 1 and change by ~1e-4 per step, so FP32 state would lose ~6e-8 per step to
+// rounding; FP64 state keeps the accumulated BOLD error at the increment's
+// relative precision.
+template <> struct BW<float> {
+    double x = 0.0, f = 1.0, nu = 1.0, q = 1.0;
+    __device__ __forceinline__ void reset() { x = 0.0; f = 1.0; nu = 1.0; q = 1.0; }
+    __device__ __forceinline__ void step(float SE) {
+        constexpr float mdt = (float)C64::model_dt;
+        constexpr float mdt_itau = (float)(C64::model_dt * C64::itau);
+        constexpr float kap = (float)C64::kappa, yy = (float)C64::y;
+        constexpr float ia = (float)C64::ialpha, irho = (float)(1.0 / C64::rho);
+        const float xf = (float)x, fm1 = (float)(f - 1.0), ff = (float)f;
+        x += (double)(mdt * (SE - kap * xf - yy * fm1));
+        const double f_new = f + (double)(mdt * (float)x);
+        const float nu_ia = __builtin_amdgcn_exp2f(ia * __builtin_amdgcn_logf((float)nu));
+        nu += (double)(mdt_itau * (ff - nu_ia));
+        const float nuf = (float)nu;
+        const float nu_ia_new = __builtin_amdgcn_exp2f(ia * __builtin_amdgcn_logf(nuf));
+        const float p = __builtin_amdgcn_exp2f((float)kLog2OneMinRho * __builtin_amdgcn_rcpf(ff));
+        q += (double)(mdt_itau * (ff * (1.0f - p) * irho -
+                                  nu_ia_new * (float)q * __builtin_amdgcn_rcpf(nuf)));
+        f = f_new;
+    }
+    __device__ __forceinline__ float bold() const {
+        return (float)(C64::V0 * (C64::k1 * (1 - q) + C64::k2 * (1 - q / nu) + C64::k3 * (1 - nu)));
+    }
+};
+
+// Numerical FIC decision for one node (bnm.cpp:239-257): returns true when the
+// node's mean I_E is out of band; adjusts wIE/delta when a trial is left.
+// FP64 delta follows the CPU path's float rounding of delta after a down-step
+// (bnm.cpp:250 fmaxf).
+template <class R>
+__device__ __forceinline__ bool fic_decide(double mean_IE_sum, unsigned duration, int trial,
+                                           int max_trials, R& wIE, R& delta) {
+    const double mean = mean_IE_sum / duration;
+    const double diff = mean - C64::b_a_ratio_E;
+    if (fabs(diff + 0.026) > 0.005) {
+        if (trial < max_trials) {
+            if (diff < -0.026) {
+                wIE -= delta;
+                delta -= (R)0.001;
+                delta = (R)fmaxf((float)delta, 0.001f);
+            } else {
+                wIE += delta;
+            }
+        }
+        return true;
+    }
+    return false;
+}
+
+// Per-launch integration settings (host fills, passed by value).
+struct SimArgs {
+    int N;                 // nodes
+    int B;                 // simulations in this launch
+    int time_steps;        // ms, loop runs ts = 0..time_steps
+    int bold_tr;           // ms
+    int output_ts;         // BOLD samples per simulation
+    int n_vols_remove;
+    int adjust_fic;        // numerical FIC on
+    int max_fic_trials;
+    int i_start, i_end;    // FIC sampling window (ms, on the post-increment clock)
+    double init_delta;
+    long long noise_period;   // ms of draws in the noise array
+    const float* noise;       // [period][10][N][2]
+    const double* G;          // [B]
+    const double* wEE;        // [B*N]
+    const double* wEI;        // [B*N]
+    const double* wIE_in;     // [B*N] initial wIE
+    double* wIE_out;          // [B*N] adjusted wIE (may alias wIE_in)
+    void* bold;               // [B][output_ts][N] in the kernel precision
+    double* ext;              // [B][6][N] means of S_E, I_E, r_E, S_I, I_I, r_I
+    int* fic_failed;          // [B]
+    int* fic_trials;          // [B]
+};
+
+}  // namespace bnm

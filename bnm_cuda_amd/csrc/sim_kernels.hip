@@ -1,0 +1,407 @@
+// sim_kernels.hip -- integration kernels: rWW substeps, Balloon-Windkessel,
+// BOLD sampling at TR, numerical FIC (bnm.cu:72-315 / bnm.cpp:143-290 of the
+// reference), one simulation per workgroup.
+//
+//  k_sim_plain<R>  one lane per node, coupling dot read from SC in global
+//                  memory (row-major, lane j walks row j), S_E of the previous
+//                  substep in LDS, two barriers per substep: the reference's
+//                  GPU design, kept as the "plain" denominator.
+//  k_sim_split<R,KC,LS> SC row slices register-resident; the coupling dot
+//                  of a node split over LS lanes of a wave and KS wave groups
+//                  (see the kernel's comment); one barrier per substep when
+//                  KS == 1.
+#include "kernels.hpp"
+#include "model.hpp"
+
+namespace bnm {
+
+namespace {
+
+template <class R> struct Vec4;
+template <> struct Vec4<float> { using T = float4; };
+template <> struct Vec4<double> { using T = double2; };
+
+// Per-lane state of one node.
+template <class R>
+struct Lane {
+    R SE = R(0.001), SI = R(0.001), IE = 0, II = 0, rE = 0, rI = 0;
+    BW<R> bw;
+    R wEE = 0, wEI = 0, wIE = 0, delta = 0;
+    double mIE = 0;
+    double ext[6] = {0, 0, 0, 0, 0, 0};
+    __device__ void reset() {
+        SE = R(0.001); SI = R(0.001);
+        bw.reset();
+        mIE = 0;
+        for (int i = 0; i < 6; ++i) ext[i] = 0;
+    }
+};
+
+// Coupling gain argument of Node<R>::step: FP64 keeps c*G*J_NMDA in the
+// reference order, FP32 folds G*J_NMDA.
+template <class R> __device__ __forceinline__ R gain(double G);
+template <> __device__ __forceinline__ double gain<double>(double G) { return G; }
+template <> __device__ __forceinline__ float gain<float>(double G) { return (float)(G * C64::J_NMDA); }
+
+template <class R>
+__device__ __forceinline__ void load_params(const SimArgs& a, int sim, int j, Lane<R>& L) {
+    const size_t o = (size_t)sim * a.N + j;
+    L.wEE = (R)a.wEE[o];
+    L.wEI = (R)a.wEI[o];
+    L.wIE = (R)a.wIE_in[o];
+    L.delta = (R)a.init_delta;
+}
+
+// BOLD sample + extended-output sums at ts % TR == 0 (bnm.cpp:212-226).
+template <class R>
+__device__ __forceinline__ void sample(const SimArgs& a, int sim, int j, int vol, Lane<R>& L) {
+    R* bold = reinterpret_cast<R*>(a.bold);
+    bold[((size_t)sim * a.output_ts + vol) * a.N + j] = L.bw.bold();
+    if (vol >= a.n_vols_remove) {
+        L.ext[0] += (double)L.SE;
+        L.ext[1] += (double)L.IE;
+        L.ext[2] += (double)L.rE;
+        L.ext[3] += (double)L.SI;
+        L.ext[4] += (double)L.II;
+        L.ext[5] += (double)L.rI;
+    }
+}
+
+template <class R>
+__device__ __forceinline__ void finish(const SimArgs& a, int sim, int j, bool owner, int vol,
+                                       int trial, bool failed, const Lane<R>& L) {
+    if (owner) {
+        const size_t o = (size_t)sim * a.N + j;
+        a.wIE_out[o] = (double)L.wIE;
+        const int n = vol - a.n_vols_remove;
+        for (int i = 0; i < 6; ++i)
+            a.ext[((size_t)sim * 6 + i) * a.N + j] = n > 0 ? L.ext[i] / n : 0.0;
+    }
+    if (threadIdx.x == 0) {
+        a.fic_failed[sim] = failed ? 1 : 0;
+        a.fic_trials[sim] = trial;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// plain: one lane per node, SC from global memory (bnm.cu:177-217 design)
+// ---------------------------------------------------------------------------
+template <class R>
+__global__ __launch_bounds__(1024) void k_sim_plain(const R* __restrict__ SC, SimArgs a) {
+#pragma clang fp contract(off)
+    const int sim = blockIdx.x;
+    const int j = threadIdx.x;
+    const int N = a.N;
+    const bool act = j < N;
+    extern __shared__ unsigned char smem_raw[];
+    R* S_prev = reinterpret_cast<R*>(smem_raw);
+
+    Lane<R> L;
+    if (act) load_params(a, sim, j, L);
+    const R gp = gain<R>(a.G[sim]);
+    if (act) S_prev[j] = R(0.001);
+    int ts = 0, vol = 0, trial = 0;
+    bool adjusting = a.adjust_fic != 0, failed = false;
+    const unsigned dur = (unsigned)(a.i_end - a.i_start + 1);
+    const float* noise = a.noise;
+    while (ts <= a.time_steps) {
+        const long long ms = ts % a.noise_period;
+        for (int s = 0; s < 10; ++s) {
+            __syncthreads();
+            if (act) {
+                R c = 0;
+                const R* row = SC + (size_t)j * N;
+                for (int k = 0; k < N; ++k) c += row[k] * S_prev[k];
+                const size_t nb = (((size_t)ms * 10 + s) * N + j) * 2;
+                Node<R>::step(c, gp, L.wEE, L.wEI, L.wIE, (R)noise[nb], (R)noise[nb + 1], L.SE,
+                              L.SI, L.IE, L.II, L.rE, L.rI);
+            }
+            __syncthreads();
+            if (act) S_prev[j] = L.SE;
+        }
+        if (act) L.bw.step(L.SE);
+        if (ts % a.bold_tr == 0) {
+            if (act) sample(a, sim, j, vol, L);
+            ++vol;
+        }
+        ++ts;
+        if (adjusting) {
+            if (act && ts >= a.i_start && ts <= a.i_end) L.mIE += (double)L.IE;
+            if (ts == a.i_end) {
+                const bool out = act && fic_decide<R>(L.mIE, dur, trial, a.max_fic_trials, L.wIE, L.delta);
+                if (__syncthreads_or(out)) {
+                    if (trial < a.max_fic_trials) {
+                        if (act) { L.reset(); S_prev[j] = R(0.001); }
+                        ts = 0; vol = 0; ++trial;
+                    } else {
+                        failed = true; adjusting = false;
+                    }
+                } else {
+                    adjusting = false;
+                }
+            }
+        }
+    }
+    finish(a, sim, j, act, vol, trial, failed, L);
+}
+
+// ---------------------------------------------------------------------------
+// split: register-resident SC slices.  The coupling dot of node j is split
+// into LS*KS k-slices of KC values: LS slices live in LS lanes of one wave
+// (summed by v_permlane32/16_swap, no LDS, no barrier) and KS slices in KS
+// wave groups (summed through LDS, one extra barrier).  S_E is double-buffered
+// in LDS, slices padded (stride SS) so the LS concurrent 16-byte broadcast
+// reads of a wave fall in distinct banks.
+// ---------------------------------------------------------------------------
+template <class R> __device__ __forceinline__ R lane_sum32(R v);
+template <class R> __device__ __forceinline__ R lane_sum16(R v);
+template <> __device__ __forceinline__ float lane_sum32<float>(float v) {
+    auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+template <> __device__ __forceinline__ float lane_sum16<float>(float v) {
+    auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+__device__ __forceinline__ double swap_f64(double v, bool s32) {
+    const unsigned long long u = __double_as_longlong(v);
+    const unsigned lo = (unsigned)u, hi = (unsigned)(u >> 32);
+    auto rl = s32 ? __builtin_amdgcn_permlane32_swap(lo, lo, false, false)
+                  : __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    auto rh = s32 ? __builtin_amdgcn_permlane32_swap(hi, hi, false, false)
+                  : __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    // element 0: the lower-row lane's value, element 1: the upper-row lane's
+    const double a = __longlong_as_double(((unsigned long long)rh[0] << 32) | rl[0]);
+    const double b = __longlong_as_double(((unsigned long long)rh[1] << 32) | rl[1]);
+    return a + b;
+}
+template <> __device__ __forceinline__ double lane_sum32<double>(double v) { return swap_f64(v, true); }
+template <> __device__ __forceinline__ double lane_sum16<double>(double v) { return swap_f64(v, false); }
+
+template <class R, int KC, int LS, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_sim_split(const R* __restrict__ SC, SimArgs a, int NW,
+                                                     int KS, int SS) {
+    using V = typename Vec4<R>::T;
+    constexpr int VW = sizeof(V) / sizeof(R);
+    constexpr int GN = 64 / LS;  // nodes per wave
+    static_assert(KC % 4 == 0, "slice length must be a multiple of 4");
+    const int sim = blockIdx.x;
+    const int N = a.N;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nw = wave % NW, ks = wave / NW;
+    const int ln = lane % GN, sl = lane / GN;
+    const int j = nw * GN + ln;
+    const int slice = ks * LS + sl;
+    const int k0 = slice * KC;
+    const bool comp = ks == 0 && j < N;  // computes the node update
+    const bool writer = comp && sl == 0; // publishes S_E, BOLD, outputs
+    const int ldS = KS * LS * SS;
+    const int posj = (j / KC) * SS + (j % KC);  // LDS slot of node j
+    extern __shared__ unsigned char smem_raw[];
+    R* Sb = reinterpret_cast<R*>(smem_raw);   // [2][ldS]
+    R* part = Sb + 2 * ldS;                   // [KS][NW*GN]
+
+    R sc[KC];
+#pragma unroll
+    for (int q = 0; q < KC; ++q) {
+        const int k = k0 + q;
+        sc[q] = (j < N && k < N) ? SC[(size_t)j * N + k] : R(0);
+    }
+    for (int i = threadIdx.x; i < 2 * ldS; i += blockDim.x) {
+        const int slot = i % ldS, qq = slot % SS, k = (slot / SS) * KC + qq;
+        Sb[i] = (qq < KC && k < N) ? R(0.001) : R(0);
+    }
+
+    Lane<R> L;
+    if (comp) load_params(a, sim, j, L);
+    const R gp = gain<R>(a.G[sim]);
+    int ts = 0, vol = 0, trial = 0, buf = 0;
+    bool adjusting = a.adjust_fic != 0, failed = false;
+    const unsigned dur = (unsigned)(a.i_end - a.i_start + 1);
+    const int P = (int)a.noise_period;
+    const float2* noise2 = reinterpret_cast<const float2*>(a.noise) + j;
+    float2 nz[10], nzn[10];
+    int ms = 0;
+    if (comp) {
+#pragma unroll
+        for (int s = 0; s < 10; ++s) nz[s] = noise2[(size_t)s * N];
+    }
+    __syncthreads();
+    while (ts <= a.time_steps) {
+        const int ms_next = ms + 1 == P ? 0 : ms + 1;
+        if (comp) {  // prefetch the next ms of noise behind this ms's work
+            const float2* p = noise2 + (size_t)ms_next * 10 * N;
+#pragma unroll
+            for (int s = 0; s < 10; ++s) nzn[s] = p[(size_t)s * N];
+        }
+#pragma unroll
+        for (int s = 0; s < 10; ++s) {
+            const V* Sr = reinterpret_cast<const V*>(Sb + buf * ldS + slice * SS);
+            R acc[4] = {0, 0, 0, 0};
+            V sv[KC / VW];  // all slice reads in flight before the FMAs
+#pragma unroll
+            for (int q = 0; q < KC / VW; ++q) sv[q] = Sr[q];
+#pragma unroll
+            for (int q = 0; q < KC / VW; ++q) {
+                const V v = sv[q];
+                if constexpr (VW == 4) {
+                    acc[0] = fma(sc[4 * q + 0], v.x, acc[0]);
+                    acc[1] = fma(sc[4 * q + 1], v.y, acc[1]);
+                    acc[2] = fma(sc[4 * q + 2], v.z, acc[2]);
+                    acc[3] = fma(sc[4 * q + 3], v.w, acc[3]);
+                } else {
+                    acc[(2 * q) & 3] = fma(sc[2 * q + 0], v.x, acc[(2 * q) & 3]);
+                    acc[(2 * q + 1) & 3] = fma(sc[2 * q + 1], v.y, acc[(2 * q + 1) & 3]);
+                }
+            }
+            R c = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+            if constexpr (LS == 4) c = lane_sum16<R>(c);
+            if constexpr (LS >= 2) c = lane_sum32<R>(c);
+            if (KS > 1) {
+                if (ks > 0 && sl == 0) part[ks * NW * GN + j] = c;
+                __syncthreads();
+                if (ks == 0)
+                    for (int t = 1; t < KS; ++t) c += part[t * NW * GN + j];
+            }
+            if (comp) {
+                Node<R>::step(c, gp, L.wEE, L.wEI, L.wIE, (R)nz[s].x, (R)nz[s].y, L.SE, L.SI, L.IE,
+                              L.II, L.rE, L.rI);
+                if (writer) Sb[(buf ^ 1) * ldS + posj] = L.SE;
+            }
+            __syncthreads();
+            buf ^= 1;
+        }
+        if (comp) L.bw.step(L.SE);
+        if (ts % a.bold_tr == 0) {
+            if (writer) sample(a, sim, j, vol, L);
+            ++vol;
+        }
+        ++ts;
+        bool restarted = false;
+        if (adjusting) {
+            if (comp && ts >= a.i_start && ts <= a.i_end) L.mIE += (double)L.IE;
+            if (ts == a.i_end) {
+                // every lane of a node holds the same state and decides alike
+                const bool out = comp && fic_decide<R>(L.mIE, dur, trial, a.max_fic_trials, L.wIE, L.delta);
+                if (__syncthreads_or(out)) {
+                    if (trial < a.max_fic_trials) {
+                        if (comp) L.reset();
+                        if (writer) Sb[buf * ldS + posj] = R(0.001);
+                        ts = 0; vol = 0; ++trial;
+                        restarted = true;
+                        __syncthreads();
+                    } else {
+                        failed = true; adjusting = false;
+                    }
+                } else {
+                    adjusting = false;
+                }
+            }
+        }
+        if (restarted) {
+            ms = 0;
+            if (comp) {
+#pragma unroll
+                for (int s = 0; s < 10; ++s) nz[s] = noise2[(size_t)s * N];
+            }
+        } else {
+            ms = ms_next;
+#pragma unroll
+            for (int s = 0; s < 10; ++s) nz[s] = nzn[s];
+        }
+    }
+    finish(a, sim, j, writer, vol, trial, failed, L);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+template <class R>
+hipError_t launch_sim_plain(const R* SC, const SimArgs& a, hipStream_t st) {
+    const int threads = ((a.N + 63) / 64) * 64;
+    const size_t smem = (size_t)a.N * sizeof(R);
+    hipLaunchKernelGGL(k_sim_plain<R>, dim3(a.B), dim3(threads), smem, st, SC, a);
+    return hipGetLastError();
+}
+
+// Register budget of a lane: KC SC values, KC read-ahead S_E values, plus the
+// node state, noise double buffer and FP64 sums (~130 VGPRs in FP32, ~200 in
+// FP64); the workgroup size bound follows (4, 2 or 1 waves per SIMD).
+constexpr int maxt_for(int KC, int elem) {
+    const int regs = 2 * KC * (elem / 4) + (elem == 4 ? 130 : 200);
+    return regs <= 128 ? 1024 : regs <= 256 ? 512 : 256;
+}
+
+static const int kKC[] = {8, 16, 28, 32, 52, 64, 100, 128};
+
+FastShape fast_shape(int N, int LS, int KS, size_t elem) {
+    FastShape s{};
+    s.LS = LS;
+    s.KS = KS;
+    s.NW = (N * LS + 63) / 64;
+    const int per = (N + LS * KS - 1) / (LS * KS);
+    s.KC = 0;
+    for (int kc : kKC)
+        if (kc >= per) { s.KC = kc; break; }
+    s.kc = s.KC;
+    // slice stride: smallest SS >= KC (multiple of 4) whose LS slices start in
+    // 4-bank windows that do not overlap (32 banks of 4 bytes)
+    int SS = s.KC;
+    const int wb = (int)(elem / 4) * 4;  // banks covered by one 16-byte read
+    for (;; SS += 4) {
+        bool ok = true;
+        for (int x = 0; x < LS && ok; ++x)
+            for (int y = x + 1; y < LS && ok; ++y) {
+                const int bx = (int)((x * SS * elem / 4) % 32), by = (int)((y * SS * elem / 4) % 32);
+                const int d = (by - bx + 32) % 32;
+                if (d < wb || d > 32 - wb) ok = false;
+            }
+        if (ok || SS > s.KC + 64) break;
+    }
+    s.SS = SS;
+    s.threads = s.NW * KS * 64;
+    s.smem = (2 * (size_t)KS * LS * SS + (size_t)KS * s.NW * (64 / LS)) * elem;
+    s.ok = s.KC > 0 && s.threads <= maxt_for(s.KC, (int)elem) && s.smem <= 64 * 1024 &&
+           (LS == 1 || LS == 2 || LS == 4);
+    return s;
+}
+
+template <class R, int KC, int LS>
+static void launch_split(const R* SC, const SimArgs& a, const FastShape& s, hipStream_t st) {
+    constexpr int MAXT = maxt_for(KC, (int)sizeof(R));
+    hipLaunchKernelGGL((k_sim_split<R, KC, LS, MAXT>), dim3(a.B), dim3(s.threads), s.smem, st, SC, a,
+                       s.NW, s.KS, s.SS);
+}
+
+template <class R, int KC>
+static void launch_split_kc(const R* SC, const SimArgs& a, const FastShape& s, hipStream_t st) {
+    if (s.LS == 1) launch_split<R, KC, 1>(SC, a, s, st);
+    else if (s.LS == 2) launch_split<R, KC, 2>(SC, a, s, st);
+    else launch_split<R, KC, 4>(SC, a, s, st);
+}
+
+template <class R>
+hipError_t launch_sim_fast(const R* SC, const SimArgs& a, int LS, int KS, hipStream_t st) {
+    const FastShape s = fast_shape(a.N, LS, KS, sizeof(R));
+    if (!s.ok) return hipErrorInvalidValue;
+    switch (s.KC) {
+        case 8: launch_split_kc<R, 8>(SC, a, s, st); break;
+        case 16: launch_split_kc<R, 16>(SC, a, s, st); break;
+        case 28: launch_split_kc<R, 28>(SC, a, s, st); break;
+        case 32: launch_split_kc<R, 32>(SC, a, s, st); break;
+        case 52: launch_split_kc<R, 52>(SC, a, s, st); break;
+        case 64: launch_split_kc<R, 64>(SC, a, s, st); break;
+        case 100: launch_split_kc<R, 100>(SC, a, s, st); break;
+        default: launch_split_kc<R, 128>(SC, a, s, st); break;
+    }
+    return hipGetLastError();
+}
+
+template hipError_t launch_sim_plain<double>(const double*, const SimArgs&, hipStream_t);
+template hipError_t launch_sim_plain<float>(const float*, const SimArgs&, hipStream_t);
+template hipError_t launch_sim_fast<double>(const double*, const SimArgs&, int, int, hipStream_t);
+template hipError_t launch_sim_fast<float>(const float*, const SimArgs&, int, int, hipStream_t);
+
+}  // namespace bnm

@@ -1,0 +1,184 @@
+/*
+ * bnm_hip.h -- C ABI of the MI355X (gfx950) batched reduced-Wong-Wang
+ * simulation library (libbnm_hip.so).
+ *
+ * The library integrates reduced Wong-Wang E/I node dynamics (Euler-Maruyama,
+ * dt = 0.1 ms, coupling G*SC*S_E), adjusts per-node wIE by numerical feedback
+ * inhibition control (FIC), samples Balloon-Windkessel BOLD at TR, and reduces
+ * BOLD to FC (Pearson), sliding-window FCD and the KS-based goodness of fit, for
+ * a batch of simulations that share one SC and one noise array.
+ *
+ * Every entry point is synchronous, plain C, takes flat arrays and returns a
+ * status code (BNM_OK or a negative BNM_E_*); nothing calls exit().  Array
+ * layouts are row-major:
+ *   SC            [N*N]        row j = inputs of node j (SC[j*N+k] weights S_E[k])
+ *   per-sim lists [B] or [B*N] (sim-major)
+ *   noise         [P*10*N*2]   float N(0,1), P ms of draws, order
+ *                              [ms][substep][node][E,I]; read at (ms mod P)
+ *   BOLD          [B*output_ts*N] (time-major per sim)
+ *   FC            [B*n_pairs]  lower triangle i>j row-major, optionally without
+ *                              left-right pairs (nodes 0..N/2-1 = left)
+ *   FCD           [B*n_window_pairs] lower triangle wi>wj row-major
+ */
+#ifndef BNM_HIP_H
+#define BNM_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define BNM_OK 0
+#define BNM_E_ARG (-1)      /* invalid argument or size                      */
+#define BNM_E_HIP (-2)      /* a HIP runtime call failed                     */
+#define BNM_E_NODEV (-3)    /* no gfx950 device visible                      */
+#define BNM_E_SIZE (-4)     /* empirical FC/FCD size does not fit the config */
+#define BNM_E_UNSUP (-5)    /* configuration not supported by the kernel    */
+
+/* integration precision */
+#define BNM_PREC_F64 0      /* FP64 state and coupling                      */
+#define BNM_PREC_F32 1      /* FP32 state and coupling, FP64 FIC sums       */
+#define BNM_PREC_BF16 2     /* FP32 state, BF16 coupling operands, FP32 acc */
+
+/* integration kernel */
+#define BNM_KERNEL_AUTO 0   /* autotuned choice                               */
+#define BNM_KERNEL_PLAIN 1  /* one simulation per workgroup, one lane per node,
+                               SC read from global memory (the reference's
+                               GPU design, bnm.cu:72-315)                      */
+#define BNM_KERNEL_FAST 2   /* one simulation per workgroup, SC row slices in
+                               registers, coupling split over waves            */
+#define BNM_KERNEL_MFMA 3   /* several simulations per workgroup, coupling
+                               SC[NxN] x S_E[Nxb] on MFMA                      */
+
+/* FC / FCD Gram precision */
+#define BNM_FC_F64 0
+#define BNM_FC_F32 1
+#define BNM_FC_BF16 2
+
+typedef struct bnm_config {
+    int nodes;                 /* N                                               */
+    int n_sims;                /* B                                               */
+    int time_steps;            /* simulated ms; the loop runs ts = 0..time_steps  */
+    int bold_tr;               /* TR in ms                                        */
+    int n_vols_remove;         /* volumes dropped before FC/FCD and the means     */
+    int window_size;           /* FCD window (window holds ws/2*2+1 volumes)      */
+    int window_step;           /* FCD window step (volumes)                       */
+    int exc_interhemispheric;  /* drop left-right pairs from FC/FCD               */
+    int do_fic;                /* wIE comes from w_IE_fic (else from w_IE_list)   */
+    int adjust_fic;            /* numerical FIC adjustment                        */
+    int max_fic_trials;        /* restarts allowed before FIC is declared failed  */
+    int i_sampling_start;      /* ms; mean I_E sampling window for FIC            */
+    int i_sampling_end;        /* ms                                              */
+    double init_delta;         /* initial wIE step                                */
+    long long noise_period_ms; /* ms of draws held in the noise array             */
+    int calculate_fic_penalty; /* fill fic_penalties                              */
+    int sim_only;              /* skip FC/FCD/fit (cost = max_cost)               */
+    int no_fcd;                /* skip FCD and KS                                 */
+    int use_fc_diff;           /* add |mean(emp FC) - mean(sim FC)| to the cost   */
+    double max_cost;           /* cost when FC/FCD has a NaN                      */
+    double fic_penalty_scale;
+    int fic_reject_failed;     /* +1 penalty when FIC failed                      */
+    int analytical_fic;        /* 1: compute the analytical FIC wIE on the device
+                                  into w_IE_fic before integrating; 0: take
+                                  w_IE_fic as given                               */
+    int precision;             /* BNM_PREC_*                                      */
+    int fc_precision;          /* BNM_FC_*                                        */
+    int kernel;                /* BNM_KERNEL_*                                    */
+    int sims_per_block;        /* MFMA kernel: simulations per workgroup (0=auto) */
+} bnm_config;
+
+/* Fills `cfg` with the reference defaults (constants.h:93-116), honouring the
+ * same BNM_* environment variables (BNM_BOLD_REMOVE, BNM_MAX_COST,
+ * BNM_I_SAMPLING_START/END, BNM_NUMERICAL_FIC, BNM_MAX_FIC_TRIALS_CMAES,
+ * BNM_INIT_DELTA, BNM_USE_FC_DIFF, BNM_EXC_INTERHEM,
+ * BNM_CMAES_FIC_PENALTY_SCALE, BNM_CMAES_FIC_REJECT_FAILED).  The caller then
+ * sets nodes, n_sims, time_steps, bold_tr, window_size, window_step and
+ * noise_period_ms; n_vols_remove is derived from bold_tr by bnm_finalize_config. */
+int bnm_default_config(bnm_config* cfg);
+
+/* Derives n_vols_remove = BNM_BOLD_REMOVE*1000/bold_tr (run_CMAES.cpp:483)
+ * when it is negative, and validates the config. */
+int bnm_finalize_config(bnm_config* cfg);
+
+/* Derived sizes (run_CMAES.cpp:476-594). */
+int bnm_output_ts(int time_steps, int bold_tr);
+int bnm_n_pairs(int nodes, int exc_interhemispheric);
+int bnm_n_windows(const bnm_config* cfg);
+/* Window start/end volume indices (absolute, inclusive); arrays of
+ * bnm_n_windows(cfg) ints. */
+int bnm_window_bounds(const bnm_config* cfg, int* starts, int* ends);
+
+/* Host noise array as the reference's driver makes it (run_CMAES.cpp:640-657):
+ * `count` draws of std::normal_distribution<float>(0,1) over std::mt19937(seed). */
+int bnm_make_noise(unsigned seed, long long count, float* out);
+
+/* Large noise arrays (a full-length array holds (T+1)*10*N*2 draws, 902 M at
+ * N=100 and 450 s) in parallel: draw i comes from chunk c = i / 2^20, whose
+ * std::mt19937 is seeded with std::seed_seq{seed, c}, then
+ * std::normal_distribution<float>(0,1).  The result does not depend on
+ * `threads` (<= 0: all hardware threads).  No counterpart in the reference,
+ * whose driver draws one sequential stream (run_CMAES.cpp:640-657). */
+int bnm_make_noise_chunked(unsigned seed, long long count, float* out, int threads);
+
+/* Replaces run_simulations (GPU) of bnm.cu:594-617.  Host arrays in, host
+ * arrays out; outputs that are NULL are not produced.
+ *   corr[B]            goodness-of-fit cost 1 - r(FC) + KS(FCD) [+ |dmean FC|]
+ *   fic_penalties[B]   FIC penalty (bnm.cu:752-767)
+ *   G_list[B], w_EE_list[B*N], w_EI_list[B*N], w_IE_list[B*N]
+ *   w_IE_fic[B*N]      in: FIC initial wIE (unless cfg->analytical_fic);
+ *                      out: wIE after numerical adjustment
+ *   SC[N*N], noise[noise_len], emp_FC_tril[n_emp_fc], emp_FCD_tril[n_emp_fcd]
+ *   BOLD[B*output_ts*N]
+ *   S_E, I_E, r_E, S_I, I_I, r_I [B*N]  means over TR samples after n_vols_remove
+ *   FIC_failed[B], fic_n_trials[B]
+ *   fc_trils[B*n_pairs], fcd_trils[B*n_window_pairs]                          */
+int bnm_run_simulations(const bnm_config* cfg, double* corr, double* fic_penalties,
+                        const double* G_list, const double* w_EE_list, const double* w_EI_list,
+                        const double* w_IE_list, double* w_IE_fic, const double* SC,
+                        const float* noise, long long noise_len,
+                        const double* emp_FC_tril, int n_emp_fc,
+                        const double* emp_FCD_tril, int n_emp_fcd,
+                        double* BOLD, double* S_E, double* I_E, double* r_E,
+                        double* S_I, double* I_I, double* r_I,
+                        int* FIC_failed, int* fic_n_trials,
+                        double* fc_trils, double* fcd_trils);
+
+/* ---- persistent context: device buffers kept across calls (a CMA-ES run
+ * calls run_simulations once per generation with the same SC and noise) ---- */
+typedef struct bnm_ctx bnm_ctx;
+
+/* Allocates every device buffer for `cfg` on the current HIP device. */
+bnm_ctx* bnm_ctx_create(const bnm_config* cfg, int* status);
+void bnm_ctx_destroy(bnm_ctx* ctx);
+/* Uploads inputs (host pointers). */
+int bnm_ctx_set_sc(bnm_ctx* ctx, const double* SC);
+int bnm_ctx_set_noise(bnm_ctx* ctx, const float* noise, long long noise_len);
+int bnm_ctx_set_empirical(bnm_ctx* ctx, const double* emp_FC_tril, int n_emp_fc,
+                          const double* emp_FCD_tril, int n_emp_fcd);
+int bnm_ctx_set_params(bnm_ctx* ctx, const double* G_list, const double* w_EE_list,
+                       const double* w_EI_list, const double* w_IE_list,
+                       const double* w_IE_fic);
+/* Runs the whole path on device-resident inputs; returns when done. */
+int bnm_ctx_run(bnm_ctx* ctx);
+/* Copies outputs to host (NULL outputs skipped). */
+int bnm_ctx_get_outputs(bnm_ctx* ctx, double* corr, double* fic_penalties, double* w_IE_fic,
+                        double* BOLD, double* S_E, double* I_E, double* r_E, double* S_I,
+                        double* I_I, double* r_I, int* FIC_failed, int* fic_n_trials,
+                        double* fc_trils, double* fcd_trils);
+/* Device time (ms) of the last bnm_ctx_run, per phase:
+ * [0] analytical FIC, [1] integration, [2] BOLD moments + FC + window FC,
+ * [3] FCD, [4] fit (KS, Pearson, penalty), [5] whole run.
+ * Returns the number of values written. */
+int bnm_ctx_timings(bnm_ctx* ctx, double* ms, int n);
+/* Name of the integration kernel variant the last run used, and its launch
+ * shape (for reports). */
+const char* bnm_ctx_kernel_name(bnm_ctx* ctx);
+
+/* Last error message of this thread (static storage). */
+const char* bnm_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BNM_HIP_H */

@@ -1,0 +1,178 @@
+"""GPU parity: the HIP integration + FC/FCD/fit path against the CPU oracle
+(FP64 restatement of bnm.cpp / check_fit.cpp / fic.cpp) on identical inputs
+and the identical noise array.
+
+Bars:
+* FP64 kernels: per-TR BOLD within 1e-6 relative (max |gpu - oracle| over the
+  run divided by max |oracle BOLD| of that simulation); FIC trial counts and
+  failure flags identical; FC, FCD and cost within 1e-9.
+* FP32 kernels: mean r_E within 0.05 Hz, FC within 1e-3, FCD-KS within 0.01
+  of the FP64 oracle (summary statistics; FP32 trajectories drift).
+"""
+import numpy as np
+import pytest
+
+import bnm_cuda_amd as bnm
+from bnm_cuda_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+N = 20
+B = 6
+T_MS = 60_000
+NVR = 10
+WS, STEP = 10, 2
+
+
+def _case(noise_period=T_MS + 1, seed_sc=0, max_trials=3):
+    SC = W.synthetic_sc(N, 0.3, seed_sc)
+    G = np.array([0.5, 1.0, 2.0, 0.3, 1.5, 3.0])[:B]
+    ee = np.array([0.21, 0.15, 0.3, 0.5, 0.1, 0.25])[:B]
+    ei = np.array([0.15, 0.2, 0.1, 0.4, 0.3, 0.05])[:B]
+    wEE = W.expand_homogeneous(ee, N)
+    wEI = W.expand_homogeneous(ei, N)
+    noise = bnm.make_noise(1, noise_period * 10 * N * 2)
+    rng = np.random.default_rng(7)
+    np_ = bnm.n_pairs(N, True)
+    emp_fc = rng.uniform(-0.2, 0.8, np_)
+    emp_fcd = rng.uniform(-0.5, 1.0, 190)
+    return dict(SC=SC, G=G, wEE=wEE, wEI=wEI, noise=noise, period=noise_period,
+                emp_fc=emp_fc, emp_fcd=emp_fcd, max_trials=max_trials)
+
+
+def _cfg(c, precision, kernel, fc_precision=None, **kw):
+    cfg = bnm.default_config(
+        nodes=N, n_sims=B, time_steps=T_MS, bold_tr=1000, n_vols_remove=NVR, window_size=WS,
+        window_step=STEP, noise_period_ms=c["period"], max_fic_trials=c["max_trials"],
+        precision=precision, kernel=kernel,
+        fc_precision=(bnm.FC_F64 if precision == bnm.PREC_F64 else bnm.FC_F32)
+        if fc_precision is None else fc_precision)
+    for k, v in kw.items():
+        setattr(cfg, k, v)
+    return cfg
+
+
+def _oracle(O, c, **kw):
+    return O.run_simulations(
+        c["SC"], c["G"], c["wEE"], c["wEI"], noise_arr=c["noise"], noise_period_ms=c["period"],
+        time_steps=T_MS, bold_tr=1000, n_vols_remove=NVR, window_size=WS, window_step=STEP,
+        emp_fc=c["emp_fc"], emp_fcd=c["emp_fcd"], max_fic_trials=c["max_trials"], **kw)
+
+
+@pytest.fixture(scope="module")
+def case():
+    return _case()
+
+
+@pytest.fixture(scope="module")
+def ref(oracle_mod, case):
+    return _oracle(oracle_mod, case)
+
+
+def _gpu(c, cfg):
+    return bnm.run_simulations(cfg, c["SC"], c["G"], c["wEE"], c["wEI"], noise=c["noise"],
+                               emp_fc=c["emp_fc"], emp_fcd=c["emp_fcd"])
+
+
+def _bold_rel(g, o):
+    return max(np.max(np.abs(g[b] - o[b])) / np.max(np.abs(o[b])) for b in range(g.shape[0]))
+
+
+def test_analytical_fic_matches_oracle(oracle_mod, case):
+    O = oracle_mod
+    cfg = _cfg(case, bnm.PREC_F64, bnm.KERNEL_PLAIN, adjust_fic=0, time_steps=12_000,
+               sim_only=1)
+    out = bnm.run_simulations(cfg, case["SC"], case["G"], case["wEE"], case["wEI"],
+                              noise=case["noise"])
+    for b in range(B):
+        w, unstable = O.analytical_fic(case["SC"], case["G"][b], case["wEE"][b], case["wEI"][b])
+        if unstable:
+            w = np.ones(N)
+        np.testing.assert_allclose(out["wIE_fic"][b], w, rtol=1e-12, atol=1e-13)
+
+
+SPLITS = [(1, 1), (2, 1), (4, 1), (1, 2), (2, 2)]
+
+
+@pytest.fixture
+def split(request, monkeypatch):
+    ls, ks = request.param
+    monkeypatch.setenv("BNM_FAST_LS", str(ls))
+    monkeypatch.setenv("BNM_FAST_KS", str(ks))
+    return request.param
+
+
+@pytest.mark.parametrize("split", SPLITS, indirect=True, ids=lambda p: f"LS{p[0]}KS{p[1]}")
+def test_f64_split_kernel_matches_oracle(case, ref, split):
+    """Every lane/wave split of the register-resident kernel reproduces the
+    FP64 oracle (a wrong cross-lane pairing would not)."""
+    g = _gpu(case, _cfg(case, bnm.PREC_F64, bnm.KERNEL_FAST))
+    np.testing.assert_array_equal(g["fic_trials"], ref["fic_trials"])
+    rel = _bold_rel(g["bold"], ref["bold"])
+    print("split", split, "per-TR BOLD max relative error", rel)
+    assert rel <= 1e-6
+    np.testing.assert_allclose(g["corr"], ref["corr"], atol=1e-9)
+
+
+@pytest.mark.parametrize("kernel", [bnm.KERNEL_PLAIN, bnm.KERNEL_FAST])
+def test_f64_bold_fic_fit_match_oracle(case, ref, kernel):
+    g = _gpu(case, _cfg(case, bnm.PREC_F64, kernel))
+    print("kernel", kernel, "trials", g["fic_trials"], ref["fic_trials"])
+    np.testing.assert_array_equal(g["fic_trials"], ref["fic_trials"])
+    np.testing.assert_array_equal(g["fic_failed"], ref["fic_failed"])
+    np.testing.assert_allclose(g["wIE_fic"], ref["wIE_fic"], rtol=1e-12, atol=1e-12)
+    rel = _bold_rel(g["bold"], ref["bold"])
+    print("per-TR BOLD max relative error", rel)
+    assert rel <= 1e-6
+    np.testing.assert_allclose(g["ext"], ref["ext"], rtol=1e-7, atol=1e-9)
+    np.testing.assert_allclose(g["fc"], ref["fc"], atol=1e-9)
+    np.testing.assert_allclose(g["fcd"], ref["fcd"], atol=1e-9)
+    np.testing.assert_allclose(g["corr"], ref["corr"], atol=1e-9)
+    np.testing.assert_allclose(g["fic_penalty"], ref["fic_penalty"], atol=1e-12)
+
+
+@pytest.mark.parametrize("kernel", [bnm.KERNEL_PLAIN, bnm.KERNEL_FAST])
+def test_f32_summary_stats_within_bounds(oracle_mod, case, ref, kernel):
+    O = oracle_mod
+    g = _gpu(case, _cfg(case, bnm.PREC_F32, kernel))
+    d_re = np.max(np.abs(g["ext"][:, 2] - ref["ext"][:, 2]))
+    d_fc = np.max(np.abs(g["fc"] - ref["fc"]))
+    d_ks = max(abs(O.ks(case["emp_fcd"], g["fcd"][b]) - O.ks(case["emp_fcd"], ref["fcd"][b]))
+               for b in range(B))
+    rel = _bold_rel(g["bold"], ref["bold"])
+    print(f"kernel {kernel}: trials gpu {g['fic_trials']} oracle {ref['fic_trials']}; "
+          f"max|d mean rE| {d_re:.3g} Hz, max|d FC| {d_fc:.3g}, max|d KS| {d_ks:.3g}, "
+          f"per-TR BOLD rel {rel:.3g}")
+    for b in range(B):
+        print(f"  sim {b}: |dFC| {np.max(np.abs(g['fc'][b] - ref['fc'][b])):.3g} "
+              f"BOLD rel {np.max(np.abs(g['bold'][b] - ref['bold'][b])) / np.max(np.abs(ref['bold'][b])):.3g} "
+              f"failed {ref['fic_failed'][b]}")
+    assert d_re <= 0.05
+    assert d_fc <= 1e-3
+    assert d_ks <= 0.01
+
+
+def test_wrapped_noise_and_no_fic(oracle_mod):
+    """Noise array of 1000 ms read at (ms mod 1000), and FIC off (wIE from w_IE_list)."""
+    O = oracle_mod
+    c = _case(noise_period=1000)
+    wIE = np.full((B, N), 1.0)
+    o = O.run_simulations(c["SC"], c["G"], c["wEE"], c["wEI"], wIE, noise_arr=c["noise"],
+                          noise_period_ms=c["period"], time_steps=T_MS, bold_tr=1000,
+                          n_vols_remove=NVR, window_size=WS, window_step=STEP,
+                          emp_fc=c["emp_fc"], emp_fcd=c["emp_fcd"], do_fic=False)
+    cfg = _cfg(c, bnm.PREC_F64, bnm.KERNEL_FAST, do_fic=0)
+    g = bnm.run_simulations(cfg, c["SC"], c["G"], c["wEE"], c["wEI"], wIE, noise=c["noise"],
+                            emp_fc=c["emp_fc"], emp_fcd=c["emp_fcd"])
+    assert _bold_rel(g["bold"], o["bold"]) <= 1e-6
+    assert not g["fic_trials"].any()
+
+
+def test_fic_max_trials_zero_marks_failure(oracle_mod):
+    O = oracle_mod
+    c = _case(max_trials=0)
+    o = _oracle(O, c)
+    g = _gpu(c, _cfg(c, bnm.PREC_F64, bnm.KERNEL_FAST))
+    np.testing.assert_array_equal(g["fic_failed"], o["fic_failed"])
+    assert not g["fic_trials"].any()
+    np.testing.assert_allclose(g["wIE_fic"], o["wIE_fic"], rtol=1e-12)
