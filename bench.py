@@ -1,0 +1,243 @@
+#!/usr/bin/env python3
+"""Benchmark: simulated seconds per wall second of the batched rWW -> BOLD ->
+FC/FCD/KS path on one MI355X.
+
+Workload (pop128): 128 parameter sets drawn uniformly (seed 3) from
+G in [0.5, 4], wEE, wEI in [0.05, 0.75]; N = 100 synthetic SC (symmetric, zero
+diagonal, 30 % log-normal(0,1) weights, mean row sum 1, seed 0); analytical +
+numerical FIC (up to 10 trials); 450 s at dt = 0.1 ms, BOLD every 1 ms, TR 1 s,
+30 s discarded; FCD window 30 / step 5 (79 windows, 3081 FCD values); noise
+N(0,1) float draws covering the whole run (seed 1); empirical target = FC/FCD
+of the single-simulation config (G 0.5, wEE 0.21, wEI 0.15) on a second SC
+(seed 2), simulated once by the FP64 path before timing.
+
+A step is one bnm_ctx_run: analytical FIC, integration (FP32 register-resident
+split kernel), FC, windowed FC, FCD and the KS fit for all 128 simulations, on
+inputs already resident in HBM.  `value` counts nominal simulated seconds
+(128 x 450 s per step); FIC restarts re-simulate 10 s each and are reported
+separately.
+
+Prints ONE JSON line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import bnm_cuda_amd as bnm  # noqa: E402
+from bnm_cuda_amd import workloads as W  # noqa: E402
+
+PEAK_FP32_TFLOPS = 157.3   # MI355X dense FP32 matrix peak
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_inputs(N, B, T, seed_sc=0, seed_pop=3, seed_noise=1):
+    SC = W.synthetic_sc(N, 0.3, seed_sc)
+    g, ee, ei = W.population(B, seed_pop)
+    noise = W.noise(N, T + 1, seed_noise)
+    return SC, g, W.expand_homogeneous(ee, N), W.expand_homogeneous(ei, N), noise
+
+
+def cfg_for(N, B, T, precision, kernel, **kw):
+    return bnm.default_config(nodes=N, n_sims=B, time_steps=T, bold_tr=1000, window_size=30,
+                              window_step=5, noise_period_ms=T + 1, precision=precision,
+                              kernel=kernel,
+                              fc_precision=bnm.FC_F64 if precision == bnm.PREC_F64 else bnm.FC_F32,
+                              **kw)
+
+
+def empirical_target(N, T, noise):
+    """FC/FCD of the single-simulation config on SC seed 2 (FP64 path)."""
+    cfg = cfg_for(N, 1, T, bnm.PREC_F64, bnm.KERNEL_FAST)
+    SC2 = W.synthetic_sc(N, 0.3, 2)
+    np_ = bnm.n_pairs(N, True)
+    with bnm.Context(cfg) as c:
+        c.set_sc(SC2)
+        c.set_noise(noise)
+        c.set_empirical(np.zeros(np_), np.zeros(1))  # placeholder target
+        c.set_params([0.5], np.full((1, N), 0.21), np.full((1, N), 0.15))
+        c.run()
+        o = c.outputs(bold=False)
+    return o["fc"][0], o["fcd"][0]
+
+
+def ks_np(x, y):
+    x = np.sort(x)
+    y = np.sort(y)
+    z = np.concatenate([x, y])
+    return float(np.max(np.abs(np.searchsorted(x, z, "right") / x.size -
+                               np.searchsorted(y, z, "right") / y.size)))
+
+
+def run_ctx(cfg, SC, noise, emp, params, steps=1, warmup=0):
+    c = bnm.Context(cfg)
+    c.set_sc(SC)
+    c.set_noise(noise)
+    c.set_empirical(*emp)
+    c.set_params(*params)
+    for _ in range(warmup):
+        c.run()
+    t0 = time.perf_counter()
+    integ = []
+    for _ in range(steps):
+        c.run()
+        integ.append(c.timings())
+    wall = time.perf_counter() - t0
+    out = c.outputs(bold=True)
+    name = c.kernel_name()
+    c.close()
+    return wall, integ, out, name
+
+
+def cpu_baseline(SC, params, noise, N, T_sample, n_sims, threads):
+    """The CPU oracle (FP64, one simulation per OpenMP thread, scalar N^2
+    coupling dot) on a bounded sample of the same workload."""
+    from oracle import oracle as O  # test infrastructure, checker/baseline only
+    O.build()
+    g, ee, ei = params
+    n = N * (T_sample + 1) * 20
+    t0 = time.perf_counter()
+    r = O.run_simulations(SC, g[:n_sims], ee[:n_sims], ei[:n_sims], noise_arr=noise[:n],
+                          noise_period_ms=T_sample + 1, time_steps=T_sample, bold_tr=1000,
+                          n_vols_remove=30, window_size=30, window_step=5,
+                          emp_fc=np.zeros(bnm.n_pairs(N, True)), emp_fcd=np.zeros(1),
+                          sim_only=True, want_bold=False, n_threads=threads)
+    wall = time.perf_counter() - t0
+    integrated = n_sims * (T_sample + 1) / 1000 + 10.0 * int(r["fic_trials"].sum())
+    return wall, integrated
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--duration-ms", type=int, default=450_000)
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 reference runs")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--cpu-sample-ms", type=int, default=60_000)
+    ap.add_argument("--cpu-sims", type=int, default=16)
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE "
+              f"{world}): simulations are independent, so a population would shard over GPUs "
+              "with no per-step exchange, but this build measures the one-GPU share.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    N, B, T = 100, 128, args.duration_ms
+    log(f"[bench] generating inputs: N={N} B={B} T={T} ms, noise {(T + 1) * 20 * N / 1e6:.0f} M draws")
+    t0 = time.perf_counter()
+    SC, g, wEE, wEI, noise = make_inputs(N, B, T)
+    log(f"[bench] inputs ready in {time.perf_counter() - t0:.1f} s")
+    emp = empirical_target(N, T, noise)
+    log("[bench] empirical target ready")
+    params = (g, wEE, wEI)
+
+    # ---- timed FP32 batched path --------------------------------------------
+    cfg32 = cfg_for(N, B, T, bnm.PREC_F32, bnm.KERNEL_AUTO)
+    wall, tm, o32, kname = run_ctx(cfg32, SC, noise, emp, params, args.steps, args.warmup)
+    ms_step = wall / args.steps * 1e3
+    nominal = B * T / 1000.0
+    value = nominal * args.steps / wall
+    trials = o32["fic_trials"]
+    integrated_per_step = (B * (T + 1) + 10_000 * int(trials.sum())) / 1000.0
+    integ_ms = float(np.mean([t["integrate"] for t in tm]))
+    log(f"[bench] FP32 {kname}: {ms_step:.1f} ms/step, {value:.0f} sim-s/s, "
+        f"FIC trials {int(trials.sum())} (max {int(trials.max())}), phases {tm[-1]}")
+
+    # roofline of the integration kernel: coupling FLOPs actually integrated
+    # (2*N^2 per substep, 10 substeps per ms, every simulated ms incl. FIC
+    # restarts) over the kernel's HIP-event time
+    flops = 2.0 * N * N * 10 * (B * (T + 1) + 10_000 * int(trials.sum()))
+    achieved = flops / (integ_ms * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 5), "traffic": None,
+                "kernel": kname, "kernel_ms": round(integ_ms, 2)}
+
+    extra = {}
+    # ---- FP64 reference runs: error bars and the plain-path ratio -------------
+    if not args.no_fp64:
+        cfg64 = cfg_for(N, B, T, bnm.PREC_F64, bnm.KERNEL_FAST)
+        w64, tm64, o64, k64 = run_ctx(cfg64, SC, noise, emp, params, 1, 0)
+        log(f"[bench] FP64 {k64}: {w64 * 1e3:.0f} ms")
+        cfgp = cfg_for(N, B, T, bnm.PREC_F64, bnm.KERNEL_PLAIN)
+        wp, tmp_, op, kp = run_ctx(cfgp, SC, noise, emp, params, 1, 0)
+        log(f"[bench] FP64 {kp}: {wp * 1e3:.0f} ms")
+        same_trials = o32["fic_trials"] == o64["fic_trials"]
+        d_re = np.abs(o32["ext"][:, 2] - o64["ext"][:, 2]).max(axis=1)
+        d_fc = np.abs(o32["fc"] - o64["fc"]).max(axis=1)
+        ks32 = np.array([ks_np(emp[1], o32["fcd"][b]) for b in range(B)])
+        ks64 = np.array([ks_np(emp[1], o64["fcd"][b]) for b in range(B)])
+        d_ks = np.abs(ks32 - ks64)
+        v60 = 61
+        bold_rel = np.array([np.abs(o32["bold"][b, :v60] - o64["bold"][b, :v60]).max() /
+                             np.abs(o64["bold"][b, :v60]).max() for b in range(B)])
+        d_corr = np.abs(o32["corr"] - o64["corr"])
+        extra["fp64"] = {
+            "split_f64_sim_s_per_s": round(nominal / w64, 1),
+            "plain_f64_sim_s_per_s": round(nominal / wp, 1),
+            "speedup_vs_plain_f64": round(value / (nominal / wp), 2),
+            "speedup_vs_split_f64": round(value / (nominal / w64), 2),
+            "plain_f64_kernel": kp, "split_f64_kernel": k64,
+        }
+        extra["error_vs_fp64"] = {
+            "sims_same_fic_trials": int(same_trials.sum()),
+            "mean_rE_max_abs_hz": float(d_re.max()),
+            "mean_rE_max_abs_hz_same_trials": float(d_re[same_trials].max()) if same_trials.any() else None,
+            "fc_max_abs": float(d_fc.max()),
+            "fc_max_abs_same_trials": float(d_fc[same_trials].max()) if same_trials.any() else None,
+            "fc_median_of_max_abs": float(np.median(d_fc)),
+            "fcd_ks_max_abs": float(d_ks.max()),
+            "cost_max_abs": float(d_corr.max()),
+            "bold_first60s_max_rel": float(bold_rel.max()),
+            "bold_first60s_median_rel": float(np.median(bold_rel)),
+            "bounds": {"mean_rE_hz": 0.05, "fc": 1e-3, "fcd_ks": 0.01},
+        }
+        log(f"[bench] errors vs FP64: {extra['error_vs_fp64']}")
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        cw, integ = cpu_baseline(SC, params, noise, N, args.cpu_sample_ms, args.cpu_sims, threads)
+        cpu = {"value": round(integ / cw, 2), "unit": "simulated s / wall s (incl. FIC restarts)",
+               "cores": threads, "kind": "port",
+               "sample": f"oracle (FP64 C++ restatement of bnm.cpp, OpenMP, one sim per thread) on "
+                         f"the first {args.cpu_sims} simulations of pop128 for "
+                         f"{args.cpu_sample_ms / 1000:.0f} s each, integration only "
+                         f"({integ:.0f} simulated s in {cw:.1f} s)"}
+        extra["gpu_integrated_sim_s_per_s"] = round(integrated_per_step * args.steps / wall, 1)
+        log(f"[bench] CPU baseline: {cpu}")
+
+    line = {
+        "metric": "simulated sec/s (N=100, 128 sims, FP32 vs FP64) + FC/FCD-KS error vs FP64",
+        "value": round(value, 1), "unit": "simulated s / wall s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "pop128: 128 sims x 450 s, N=100, TR 1 s, FCD window 30 step 5, "
+                               "analytical+numerical FIC, full-length noise",
+                   "nodes": N, "n_sims": B, "duration_s": T / 1000, "kernel": kname,
+                   "fic_trials_per_step": int(trials.sum()),
+                   "integrated_sim_s_per_step": integrated_per_step},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()
