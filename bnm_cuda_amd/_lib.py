@@ -28,7 +28,7 @@ FC_F64, FC_F32, FC_BF16 = 0, 1, 2
 # every symbol include/bnm_hip.h declares
 EXPORTS = (
     "bnm_default_config", "bnm_finalize_config", "bnm_output_ts", "bnm_n_pairs", "bnm_n_windows",
-    "bnm_window_bounds", "bnm_make_noise", "bnm_make_noise_chunked", "bnm_run_simulations", "bnm_ctx_create",
+    "bnm_window_bounds", "bnm_regional_params", "bnm_make_noise", "bnm_make_noise_chunked", "bnm_run_simulations", "bnm_ctx_create",
     "bnm_ctx_destroy", "bnm_ctx_set_sc", "bnm_ctx_set_noise", "bnm_ctx_set_empirical",
     "bnm_ctx_set_params", "bnm_ctx_run", "bnm_ctx_get_outputs", "bnm_ctx_timings",
     "bnm_ctx_kernel_name", "bnm_last_error",
@@ -77,6 +77,7 @@ def lib():
     L.bnm_n_windows.argtypes = [pc]
     L.bnm_window_bounds.argtypes = [pc, pi, pi]
     L.bnm_make_noise.argtypes = [C.c_uint, C.c_longlong, pf]
+    L.bnm_regional_params.argtypes = [C.c_int, C.c_int, pd, C.c_double, pd, pd]
     L.bnm_make_noise_chunked.argtypes = [C.c_uint, C.c_longlong, pf, C.c_int]
     L.bnm_run_simulations.argtypes = [pc, pd, pd, pd, pd, pd, pd, pd, pd, pf, C.c_longlong,
                                       pd, C.c_int, pd, C.c_int] + [pd] * 7 + [pi, pi, pd, pd]
@@ -138,6 +139,17 @@ def window_bounds(c: Config):
     en = np.zeros(max(n, 1), np.int32)
     lib().bnm_window_bounds(C.byref(c), _p(st, C.c_int), _p(en, C.c_int))
     return st[:n], en[:n]
+
+
+def regional_params(maps, base, scales) -> np.ndarray:
+    """Per-node parameter from maps and their coefficients (run_CMAES.cpp:61-143)."""
+    maps = np.ascontiguousarray(np.atleast_2d(maps), np.float64)
+    scales = np.ascontiguousarray(np.atleast_1d(scales), np.float64)
+    n_maps, N = maps.shape
+    out = np.zeros(N)
+    _chk(lib().bnm_regional_params(N, n_maps, _p(maps, C.c_double), float(base),
+                                   _p(scales, C.c_double), _p(out, C.c_double)))
+    return out
 
 
 def make_noise(seed: int, count: int) -> np.ndarray:

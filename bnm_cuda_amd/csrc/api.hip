@@ -185,6 +185,22 @@ int bnm_n_windows(const bnm_config* c) {
     return bnm_window_bounds(c, nullptr, nullptr);
 }
 
+int bnm_regional_params(int nodes, int n_maps, const double* maps, double base,
+                        const double* scales, double* out) {
+    if (nodes < 1 || n_maps < 0 || !out || (n_maps > 0 && (!maps || !scales)))
+        return fail(BNM_E_ARG, "bad regional parameter arguments");
+    double mn = 1000;
+    for (int j = 0; j < nodes; ++j) {
+        double scaler = 1;
+        for (int m = 0; m < n_maps; ++m) scaler += scales[m] * maps[(size_t)m * nodes + j];
+        out[j] = base * scaler;
+        if (out[j] < mn) mn = out[j];
+    }
+    if (mn < 0.001)
+        for (int j = 0; j < nodes; ++j) out[j] -= mn - 0.001;
+    return BNM_OK;
+}
+
 int bnm_make_noise(unsigned seed, long long count, float* out) {
     if (!out || count < 0) return fail(BNM_E_ARG, "bad noise buffer");
     std::mt19937 gen(seed);

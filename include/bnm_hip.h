@@ -108,6 +108,13 @@ int bnm_n_windows(const bnm_config* cfg);
  * bnm_n_windows(cfg) ints. */
 int bnm_window_bounds(const bnm_config* cfg, int* starts, int* ends);
 
+/* Regional parameters of one simulation (calculate_params, run_CMAES.cpp:61-143):
+ *   p_j = base * (1 + sum_m scales[m] * maps[m*N + j]),
+ * then, when min_j p_j < 0.001, every p_j is shifted by (0.001 - min_j p_j).
+ * maps [n_maps*N], scales [n_maps], out [N]. */
+int bnm_regional_params(int nodes, int n_maps, const double* maps, double base,
+                        const double* scales, double* out);
+
 /* Host noise array as the reference's driver makes it (run_CMAES.cpp:640-657):
  * `count` draws of std::normal_distribution<float>(0,1) over std::mt19937(seed). */
 int bnm_make_noise(unsigned seed, long long count, float* out);

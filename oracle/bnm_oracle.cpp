@@ -227,6 +227,23 @@ unsigned orc_mt19937_nth(unsigned seed, long long n) {
     return v;
 }
 
+// ---- regional parameters (run_CMAES.cpp:61-143) --------------------------
+
+// p_j = base * (1 + sum_m scale_m * map_m[j]); shift all so that min >= 0.001
+void orc_regional_params(int N, int n_maps, const double* maps, double base, const double* scales,
+                         double* out) {
+    double min_p = 1000, max_p = -1000;
+    for (int j = 0; j < N; ++j) {
+        double scaler = 1;
+        for (int m = 0; m < n_maps; ++m) scaler += scales[m] * maps[(size_t)m * N + j];
+        out[j] = base * scaler;
+        min_p = std::min(min_p, out[j]);
+        max_p = std::max(max_p, out[j]);
+    }
+    if (min_p < 0.001)
+        for (int j = 0; j < N; ++j) out[j] -= min_p - 0.001;
+}
+
 // ---- analytical FIC (fic.cpp:130-207) -----------------------------------
 
 // Root of the inhibitory-current fixed point (fic.cpp:115-119) for one node.

@@ -52,6 +52,8 @@ def lib():
         L.orc_noise.restype = None
         L.orc_mt19937_nth.argtypes = [C.c_uint, _ll]
         L.orc_mt19937_nth.restype = C.c_uint
+        L.orc_regional_params.argtypes = [_i, _i, _pd, _d, _pd, _pd]
+        L.orc_regional_params.restype = None
         L.orc_fic_inh_root.argtypes = [_d]
         L.orc_fic_inh_root.restype = _d
         L.orc_analytical_fic.argtypes = [_i, _pd, _d, _pd, _pd, _pd]
@@ -114,6 +116,14 @@ def noise(seed: int, count: int) -> np.ndarray:
 
 def mt19937_nth(seed: int, n: int) -> int:
     return lib().orc_mt19937_nth(seed, n)
+
+
+def regional_params(maps, base, scales):
+    maps = np.ascontiguousarray(np.atleast_2d(maps), np.float64)
+    scales = np.ascontiguousarray(np.atleast_1d(scales), np.float64)
+    out = np.zeros(maps.shape[1])
+    lib().orc_regional_params(maps.shape[1], maps.shape[0], maps, float(base), scales, out)
+    return out
 
 
 # ---- analytical FIC --------------------------------------------------------

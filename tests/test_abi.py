@@ -101,3 +101,14 @@ def test_context_without_gpu_fails_cleanly():
     with pytest.raises(bnm.BnmError) as e:
         bnm.Context(cfg)
     assert e.value.code == bnm.BNM_E_NODEV
+
+
+def test_regional_params_match_oracle_and_example_maps(oracle_mod):
+    maps = np.load(os.path.join(ROOT, "tests", "golden", "example_maps.npy"))
+    for base, scales in [(0.2, [0.5, -0.3, 0.1]), (0.05, [-2.0, 1.0, 0.0]), (0.3, [0, 0, 0])]:
+        a = bnm.regional_params(maps, base, scales)
+        b = oracle_mod.regional_params(maps, base, scales)
+        np.testing.assert_array_equal(a, b)
+        assert a.min() >= 0.001 - 1e-15
+    # no map: homogeneous
+    np.testing.assert_array_equal(bnm.regional_params(np.zeros((0, 7)), 0.21, []), np.full(7, 0.21))

@@ -280,3 +280,13 @@ def test_fic_penalty(oracle_mod):
     expect = 2.0 * ((1 - np.exp(-0.05)) + (1 - np.exp(-0.05 * 1.5)) + (1 - np.exp(-0.05 * 6))) / 5
     assert abs(oracle_mod.fic_penalty(m) - expect) < 1e-14
     assert abs(oracle_mod.fic_penalty(m, reject_failed=True, failed=True) - (expect + 1)) < 1e-14
+
+
+def test_regional_params_closed_form(oracle_mod):
+    maps = np.array([[1.0, -1.0, 0.5, 0.0], [0.2, 0.2, -0.4, 1.0]])
+    out = oracle_mod.regional_params(maps, 0.5, [0.3, -0.5])
+    expect = 0.5 * (1 + 0.3 * maps[0] - 0.5 * maps[1])
+    np.testing.assert_allclose(out, expect, rtol=0, atol=1e-15)
+    out = oracle_mod.regional_params(maps, 0.1, [-2.0, 0.0])  # 0.1*(1-2) < 0.001 -> shifted
+    raw = 0.1 * (1 - 2.0 * maps[0])
+    np.testing.assert_allclose(out, raw - (raw.min() - 0.001), atol=1e-15)
