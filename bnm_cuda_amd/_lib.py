@@ -21,7 +21,7 @@ BNM_E_NODEV = -3
 BNM_E_SIZE = -4
 BNM_E_UNSUP = -5
 
-PREC_F64, PREC_F32, PREC_BF16 = 0, 1, 2
+PREC_F64, PREC_F32, PREC_BF16, PREC_BF16X3 = 0, 1, 2, 3
 KERNEL_AUTO, KERNEL_PLAIN, KERNEL_FAST, KERNEL_MFMA = 0, 1, 2, 3
 FC_F64, FC_F32, FC_BF16 = 0, 1, 2
 

@@ -66,6 +66,8 @@ struct bnm_ctx {
     hipStream_t st = nullptr;
     double* SC64 = nullptr;
     float* SC32 = nullptr;
+    double* SCt64 = nullptr;  // transposed copies for the streamed large-N kernel
+    float* SCt32 = nullptr;
     float* noise = nullptr;
     long long noise_cap = 0;
     double *G = nullptr, *wEE = nullptr, *wEI = nullptr, *wIE = nullptr, *wIEfic = nullptr,
@@ -85,7 +87,7 @@ struct bnm_ctx {
     double timings[6] = {0, 0, 0, 0, 0, 0};
     std::string kname = "none";
     bool have_sc = false, have_noise = false, have_emp = false, have_params = false;
-    int fast_ls = 0, fast_ks = 0;
+    int fast_ls = 0, fast_ks = 0, stream_ks = 0, force_stream = 0;
 };
 
 extern "C" {
@@ -234,7 +236,7 @@ int bnm_make_noise_chunked(unsigned seed, long long count, float* out, int threa
 
 void bnm_ctx_destroy(bnm_ctx* x) {
     if (!x) return;
-    void* bufs[] = {x->SC64, x->SC32, x->noise, x->G, x->wEE, x->wEI, x->wIE, x->wIEfic,
+    void* bufs[] = {x->SC64, x->SC32, x->SCt64, x->SCt32, x->noise, x->G, x->wEE, x->wEI, x->wIE, x->wIEfic,
                     x->wIEscratch, x->bold, x->ext, x->failed, x->trials, x->unstable,
                     x->seg_start, x->seg_len, x->pidx, x->mean, x->inv, x->wmean, x->winv, x->fc,
                     x->wfc, x->fcd, x->emp_c, x->ks_u, x->ks_fe, x->bins, x->nanf, x->corr, x->pen};
@@ -253,6 +255,8 @@ static int ctx_alloc(bnm_ctx* x) {
     int r;
     if ((r = dalloc(&x->SC64, N * N))) return r;
     if ((r = dalloc(&x->SC32, N * N))) return r;
+    if ((r = dalloc(&x->SCt64, N * N))) return r;
+    if ((r = dalloc(&x->SCt32, N * N))) return r;
     if ((r = dalloc(&x->G, B))) return r;
     if ((r = dalloc(&x->wEE, B * N))) return r;
     if ((r = dalloc(&x->wEI, B * N))) return r;
@@ -329,7 +333,11 @@ bnm_ctx* bnm_ctx_create(const bnm_config* cfg, int* status) {
         set(fail(BNM_E_UNSUP, "the MFMA integration kernel runs in FP32/BF16 only"));
         return nullptr;
     }
-    if (c.precision == BNM_PREC_BF16 && c.kernel != BNM_KERNEL_MFMA && c.kernel != BNM_KERNEL_AUTO) {
+    if (c.precision < BNM_PREC_F64 || c.precision > BNM_PREC_BF16X3) {
+        set(fail(BNM_E_ARG, "unknown precision"));
+        return nullptr;
+    }
+    if (c.precision >= BNM_PREC_BF16 && c.kernel != BNM_KERNEL_MFMA && c.kernel != BNM_KERNEL_AUTO) {
         set(fail(BNM_E_UNSUP, "BF16 coupling needs the MFMA kernel"));
         return nullptr;
     }
@@ -348,6 +356,8 @@ bnm_ctx* bnm_ctx_create(const bnm_config* cfg, int* status) {
     }
     x->fast_ls = env_int("BNM_FAST_LS", 0);
     x->fast_ks = env_int("BNM_FAST_KS", 0);
+    x->stream_ks = env_int("BNM_STREAM_KS", 0);
+    x->force_stream = env_int("BNM_FORCE_STREAM", 0);
     if (hipStreamCreateWithFlags(&x->st, hipStreamNonBlocking) != hipSuccess) {
         set(fail(BNM_E_HIP, "hipStreamCreate failed"));
         delete x;
@@ -365,11 +375,19 @@ bnm_ctx* bnm_ctx_create(const bnm_config* cfg, int* status) {
 
 int bnm_ctx_set_sc(bnm_ctx* x, const double* SC) {
     if (!x || !SC) return fail(BNM_E_ARG, "null argument");
-    const size_t n = (size_t)x->N * x->N;
-    std::vector<float> f(n);
-    for (size_t i = 0; i < n; ++i) f[i] = (float)SC[i];
+    const size_t N = x->N, n = N * N;
+    std::vector<float> f(n), ft(n);
+    std::vector<double> dt(n);
+    for (size_t j = 0; j < N; ++j)
+        for (size_t k = 0; k < N; ++k) {
+            f[j * N + k] = (float)SC[j * N + k];
+            ft[k * N + j] = (float)SC[j * N + k];
+            dt[k * N + j] = SC[j * N + k];
+        }
     HIPCHK(hipMemcpy(x->SC64, SC, n * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(x->SC32, f.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(x->SCt64, dt.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(x->SCt32, ft.data(), n * sizeof(float), hipMemcpyHostToDevice));
     x->have_sc = true;
     return BNM_OK;
 }
@@ -452,7 +470,7 @@ int bnm_ctx_set_params(bnm_ctx* x, const double* G, const double* wEE, const dou
 static int run_integration(bnm_ctx* x, SimArgs& a) {
     const bnm_config& c = x->cfg;
     int kern = c.kernel;
-    if (kern == BNM_KERNEL_AUTO) kern = BNM_KERNEL_FAST;
+    if (kern == BNM_KERNEL_AUTO) kern = c.precision >= BNM_PREC_BF16 ? BNM_KERNEL_MFMA : BNM_KERNEL_FAST;
     char name[128];
     if (kern == BNM_KERNEL_PLAIN) {
         if (x->f64) HIPCHK(launch_sim_plain<double>(x->SC64, a, x->st));
@@ -463,19 +481,42 @@ static int run_integration(bnm_ctx* x, SimArgs& a) {
         const size_t el = x->f64 ? 8 : 4;
         int ls = x->fast_ls > 0 ? x->fast_ls : 4, ks = x->fast_ks > 0 ? x->fast_ks : 1;
         FastShape s = fast_shape(x->N, ls, ks, el);
+        if (x->force_stream) s.ok = false;
         // otherwise the first register-resident shape in order of preference
         static const int pref[][2] = {{4, 1}, {2, 1}, {1, 1}, {2, 2}, {1, 2}, {4, 2},
                                       {1, 4}, {2, 4}, {1, 8}, {2, 8}, {1, 16}};
         for (const auto& p : pref) {
-            if (s.ok) break;
+            if (s.ok || x->force_stream) break;
             s = fast_shape(x->N, p[0], p[1], el);
             if (s.ok) { ls = p[0]; ks = p[1]; }
         }
-        if (!s.ok) return fail(BNM_E_UNSUP, "no register-resident shape fits this node count");
-        if (x->f64) HIPCHK(launch_sim_fast<double>(x->SC64, a, ls, ks, x->st));
-        else HIPCHK(launch_sim_fast<float>(x->SC32, a, ls, ks, x->st));
-        std::snprintf(name, sizeof name, "split_%s(LS=%d,KS=%d,NW=%d,KC=%d,SS=%d,threads=%d)",
-                      x->f64 ? "f64" : "f32", s.LS, s.KS, s.NW, s.KC, s.SS, s.threads);
+        if (s.ok) {
+            if (x->f64) HIPCHK(launch_sim_fast<double>(x->SC64, a, ls, ks, x->st));
+            else HIPCHK(launch_sim_fast<float>(x->SC32, a, ls, ks, x->st));
+            std::snprintf(name, sizeof name, "split_%s(LS=%d,KS=%d,NW=%d,KC=%d,SS=%d,threads=%d)",
+                          x->f64 ? "f64" : "f32", s.LS, s.KS, s.NW, s.KC, s.SS, s.threads);
+        } else {
+            // SC does not fit in registers: stream it from L2, as many k-slices
+            // as the workgroup size allows
+            int sks = x->stream_ks;
+            if (sks <= 0)
+                for (sks = 16; sks > 1 && !stream_shape(x->N, sks, el).ok; sks /= 2) {}
+            const FastShape g = stream_shape(x->N, sks, el);
+            if (!g.ok) return fail(BNM_E_UNSUP, "no integration shape fits this node count");
+            if (x->f64) HIPCHK(launch_sim_stream<double>(x->SCt64, a, sks, x->st));
+            else HIPCHK(launch_sim_stream<float>(x->SCt32, a, sks, x->st));
+            std::snprintf(name, sizeof name, "stream_%s(KS=%d,NW=%d,kc=%d,threads=%d)",
+                          x->f64 ? "f64" : "f32", g.KS, g.NW, g.kc, g.threads);
+        }
+    } else if (kern == BNM_KERNEL_MFMA) {
+        if (x->f64) return fail(BNM_E_UNSUP, "the MFMA kernel integrates in FP32");
+        const int mode = c.precision == BNM_PREC_F32 ? 0 : c.precision == BNM_PREC_BF16 ? 1 : 2;
+        const MfmaShape s = mfma_shape(x->N, mode);
+        if (!s.ok) return fail(BNM_E_UNSUP, "the MFMA kernel needs N <= 128");
+        HIPCHK(launch_sim_mfma(x->SC32, a, mode, x->st));
+        static const char* mn[] = {"f32", "bf16", "bf16x3"};
+        std::snprintf(name, sizeof name, "mfma_%s(sims/block=16,waves=%d,KQ=%d,threads=%d)", mn[mode],
+                      s.waves, s.KQ, s.threads);
     } else {
         return fail(BNM_E_UNSUP, "integration kernel not available");
     }

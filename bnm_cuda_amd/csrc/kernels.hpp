@@ -25,16 +25,24 @@ template <class R>
 hipError_t launch_sim_plain(const R* SC, const SimArgs& a, hipStream_t st);
 template <class R>
 hipError_t launch_sim_fast(const R* SC, const SimArgs& a, int LS, int KS, hipStream_t st);
+// large N: SC streamed from a transposed copy SCt[k][j] in global memory
+FastShape stream_shape(int N, int KS, size_t elem);
+template <class R>
+hipError_t launch_sim_stream(const R* SCt, const SimArgs& a, int KS, hipStream_t st);
 
-// MFMA-batched integration (sim_mfma.hip): b simulations per workgroup.
+// MFMA-batched integration (sim_mfma.hip): 16 simulations per workgroup,
+// mode 0 FP32, 1 BF16, 2 BF16x3 coupling operands.
 struct MfmaShape {
-    int b;         // simulations per workgroup
+    int mode;
+    int KQ;        // K values per lane group
+    int waves;     // one 16-node row tile per wave
     int threads;
+    int LDSS;      // LDS row stride (elements)
     size_t smem;
     bool ok;
 };
-MfmaShape mfma_shape(int N, int b, bool bf16);
-hipError_t launch_sim_mfma(const float* SC, const SimArgs& a, int b, bool bf16, hipStream_t st);
+MfmaShape mfma_shape(int N, int mode);
+hipError_t launch_sim_mfma(const float* SC, const SimArgs& a, int mode, hipStream_t st);
 
 // analytical FIC on the device (fic_kernel.hip), fic.cpp:130-207
 hipError_t launch_fic_analytical(const double* SC, int N, int B, const double* G,

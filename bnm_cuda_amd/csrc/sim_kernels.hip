@@ -316,6 +316,116 @@ __global__ __launch_bounds__(MAXT) void k_sim_split(const R* __restrict__ SC, Si
 }
 
 // ---------------------------------------------------------------------------
+// split_g: the split kernel for node counts whose SC does not fit in
+// registers (N > 256).  Lane j streams its SC column slice from a transposed
+// copy SCt[k][j] in global memory (L2-resident; consecutive lanes read
+// consecutive nodes, one 256-byte line pair per wave and k), 16 k at a time
+// with the loads issued ahead of the FMAs.  LS = 1; KS wave groups split k.
+// ---------------------------------------------------------------------------
+template <class R>
+__global__ __launch_bounds__(1024) void k_sim_split_g(const R* __restrict__ SCt, SimArgs a, int NW,
+                                                      int KS, int kc) {
+    using V = typename Vec4<R>::T;
+    constexpr int VW = sizeof(V) / sizeof(R);
+    constexpr int CH = 16;  // k per chunk
+    const int sim = blockIdx.x;
+    const int N = a.N;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nw = wave % NW, ks = wave / NW;
+    const int j = nw * 64 + lane;
+    const bool comp = ks == 0 && j < N;
+    const int k0 = ks * kc;
+    const int ldS = KS * kc;  // kc multiple of 16
+    const int jc = j < N ? j : N - 1;  // clamped column for the streamed loads
+    extern __shared__ unsigned char smem_raw[];
+    R* Sb = reinterpret_cast<R*>(smem_raw);
+    R* part = Sb + 2 * ldS;
+    for (int i = threadIdx.x; i < 2 * ldS; i += blockDim.x) Sb[i] = (i % ldS) < N ? R(0.001) : R(0);
+
+    Lane<R> L;
+    if (comp) load_params(a, sim, j, L);
+    const R gp = gain<R>(a.G[sim]);
+    int ts = 0, vol = 0, trial = 0, buf = 0;
+    bool adjusting = a.adjust_fic != 0, failed = false;
+    const unsigned dur = (unsigned)(a.i_end - a.i_start + 1);
+    const float2* noise2 = reinterpret_cast<const float2*>(a.noise);
+    const int nch = kc / CH;
+    __syncthreads();
+    while (ts <= a.time_steps) {
+        const long long ms = ts % a.noise_period;
+        const float2* nzp = noise2 + (size_t)ms * 10 * N + j;
+#pragma unroll 1
+        for (int s = 0; s < 10; ++s) {
+            // this substep's noise, loaded ahead of the streamed dot
+            const float2 nz = comp ? nzp[(size_t)s * N] : float2{0.0f, 0.0f};
+            const R* Sr = Sb + buf * ldS + k0;
+            R acc[4] = {0, 0, 0, 0};
+            for (int c = 0; c < nch; ++c) {
+                const int kb = k0 + c * CH;
+                R w[CH];
+#pragma unroll
+                for (int q = 0; q < CH; ++q) {
+                    const int k = kb + q;
+                    w[q] = k < N ? SCt[(size_t)k * N + jc] : R(0);
+                }
+                const V* Sv = reinterpret_cast<const V*>(Sr + c * CH);
+#pragma unroll
+                for (int q = 0; q < CH / VW; ++q) {
+                    const V v = Sv[q];
+                    if constexpr (VW == 4) {
+                        acc[0] = fma(w[4 * q + 0], v.x, acc[0]);
+                        acc[1] = fma(w[4 * q + 1], v.y, acc[1]);
+                        acc[2] = fma(w[4 * q + 2], v.z, acc[2]);
+                        acc[3] = fma(w[4 * q + 3], v.w, acc[3]);
+                    } else {
+                        acc[(2 * q) & 3] = fma(w[2 * q + 0], v.x, acc[(2 * q) & 3]);
+                        acc[(2 * q + 1) & 3] = fma(w[2 * q + 1], v.y, acc[(2 * q + 1) & 3]);
+                    }
+                }
+            }
+            R c = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+            if (KS > 1) {
+                if (ks > 0) part[ks * NW * 64 + j] = c;
+                __syncthreads();
+                if (ks == 0)
+                    for (int t = 1; t < KS; ++t) c += part[t * NW * 64 + j];
+            }
+            if (comp) {
+                Node<R>::step(c, gp, L.wEE, L.wEI, L.wIE, (R)nz.x, (R)nz.y, L.SE, L.SI, L.IE,
+                              L.II, L.rE, L.rI);
+                Sb[(buf ^ 1) * ldS + j] = L.SE;
+            }
+            __syncthreads();
+            buf ^= 1;
+        }
+        if (comp) L.bw.step(L.SE);
+        if (ts % a.bold_tr == 0) {
+            if (comp) sample(a, sim, j, vol, L);
+            ++vol;
+        }
+        ++ts;
+        if (adjusting) {
+            if (comp && ts >= a.i_start && ts <= a.i_end) L.mIE += (double)L.IE;
+            if (ts == a.i_end) {
+                const bool out = comp && fic_decide<R>(L.mIE, dur, trial, a.max_fic_trials, L.wIE, L.delta);
+                if (__syncthreads_or(out)) {
+                    if (trial < a.max_fic_trials) {
+                        if (comp) { L.reset(); Sb[buf * ldS + j] = R(0.001); }
+                        ts = 0; vol = 0; ++trial;
+                        __syncthreads();
+                    } else {
+                        failed = true; adjusting = false;
+                    }
+                } else {
+                    adjusting = false;
+                }
+            }
+        }
+    }
+    finish(a, sim, j, comp, vol, trial, failed, L);
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 template <class R>
@@ -399,6 +509,31 @@ hipError_t launch_sim_fast(const R* SC, const SimArgs& a, int LS, int KS, hipStr
     return hipGetLastError();
 }
 
+FastShape stream_shape(int N, int KS, size_t elem) {
+    FastShape s{};
+    s.LS = 1;
+    s.KS = KS;
+    s.NW = (N + 63) / 64;
+    const int per = (N + KS - 1) / KS;
+    s.kc = ((per + 15) / 16) * 16;
+    s.KC = 0;
+    s.SS = s.kc;
+    s.threads = s.NW * KS * 64;
+    s.smem = (2 * (size_t)KS * s.kc + (size_t)KS * s.NW * 64) * elem;
+    s.ok = s.threads <= 1024 && s.smem <= 64 * 1024;
+    return s;
+}
+
+template <class R>
+hipError_t launch_sim_stream(const R* SCt, const SimArgs& a, int KS, hipStream_t st) {
+    const FastShape s = stream_shape(a.N, KS, sizeof(R));
+    if (!s.ok) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_sim_split_g<R>, dim3(a.B), dim3(s.threads), s.smem, st, SCt, a, s.NW, KS, s.kc);
+    return hipGetLastError();
+}
+
+template hipError_t launch_sim_stream<double>(const double*, const SimArgs&, int, hipStream_t);
+template hipError_t launch_sim_stream<float>(const float*, const SimArgs&, int, hipStream_t);
 template hipError_t launch_sim_plain<double>(const double*, const SimArgs&, hipStream_t);
 template hipError_t launch_sim_plain<float>(const float*, const SimArgs&, hipStream_t);
 template hipError_t launch_sim_fast<double>(const double*, const SimArgs&, int, int, hipStream_t);

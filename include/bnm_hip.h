@@ -39,6 +39,8 @@ extern "C" {
 #define BNM_PREC_F64 0      /* FP64 state and coupling                      */
 #define BNM_PREC_F32 1      /* FP32 state and coupling, FP64 FIC sums       */
 #define BNM_PREC_BF16 2     /* FP32 state, BF16 coupling operands, FP32 acc */
+#define BNM_PREC_BF16X3 3   /* FP32 state, coupling as three BF16 MFMAs of the
+                               hi/lo split operands (hi*hi + lo*hi + hi*lo)  */
 
 /* integration kernel */
 #define BNM_KERNEL_AUTO 0   /* autotuned choice                               */
@@ -47,8 +49,8 @@ extern "C" {
                                GPU design, bnm.cu:72-315)                      */
 #define BNM_KERNEL_FAST 2   /* one simulation per workgroup, SC row slices in
                                registers, coupling split over waves            */
-#define BNM_KERNEL_MFMA 3   /* several simulations per workgroup, coupling
-                               SC[NxN] x S_E[Nxb] on MFMA                      */
+#define BNM_KERNEL_MFMA 3   /* 16 simulations per workgroup, coupling
+                               SC[NxN] x S_E[Nx16] on MFMA (N <= 128)          */
 
 /* FC / FCD Gram precision */
 #define BNM_FC_F64 0

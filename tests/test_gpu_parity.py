@@ -176,3 +176,99 @@ def test_fic_max_trials_zero_marks_failure(oracle_mod):
     np.testing.assert_array_equal(g["fic_failed"], o["fic_failed"])
     assert not g["fic_trials"].any()
     np.testing.assert_allclose(g["wIE_fic"], o["wIE_fic"], rtol=1e-12)
+
+
+@pytest.mark.parametrize("prec", [bnm.PREC_F32, bnm.PREC_BF16X3], ids=["f32", "bf16x3"])
+def test_mfma_batched_kernel_summary_stats(oracle_mod, case, ref, prec):
+    """16-simulation MFMA kernel (6 real simulations in one group, 10 idle
+    columns) against the FP64 oracle on the summary-statistic bounds."""
+    O = oracle_mod
+    g = _gpu(case, _cfg(case, prec, bnm.KERNEL_MFMA))
+    d_re = np.max(np.abs(g["ext"][:, 2] - ref["ext"][:, 2]))
+    d_fc = np.max(np.abs(g["fc"] - ref["fc"]))
+    d_ks = max(abs(O.ks(case["emp_fcd"], g["fcd"][b]) - O.ks(case["emp_fcd"], ref["fcd"][b]))
+               for b in range(B))
+    rel = _bold_rel(g["bold"], ref["bold"])
+    print(f"mfma precision {prec}: trials gpu {g['fic_trials']} oracle {ref['fic_trials']}; "
+          f"max|d mean rE| {d_re:.3g} Hz, max|d FC| {d_fc:.3g}, max|d KS| {d_ks:.3g}, "
+          f"per-TR BOLD rel {rel:.3g}")
+    np.testing.assert_array_equal(g["fic_trials"], ref["fic_trials"])
+    assert d_re <= 0.05
+    assert d_fc <= 1e-3
+    assert d_ks <= 0.01
+
+
+def test_bf16_coupling_error_is_larger_than_split_bf16(case, ref):
+    """Single-BF16 coupling operands (8-bit mantissas for SC and S_E) move the
+    trajectories by more than the FC bound on this case (measured: FC 5e-2,
+    mean r_E 0.4 Hz); the hi/lo three-product form stays within it.  The test
+    records the error of both against the FP64 oracle."""
+    e = {}
+    for prec in (bnm.PREC_BF16, bnm.PREC_BF16X3):
+        g = _gpu(case, _cfg(case, prec, bnm.KERNEL_MFMA))
+        assert np.isfinite(g["bold"]).all()
+        e[prec] = np.max(np.abs(g["fc"] - ref["fc"]))
+    print(f"max |dFC| vs FP64: bf16 {e[bnm.PREC_BF16]:.3g}, bf16x3 {e[bnm.PREC_BF16X3]:.3g}")
+    assert e[bnm.PREC_BF16X3] <= 1e-3 < e[bnm.PREC_BF16]
+
+
+def test_mfma_multi_group_matches_split(case):
+    """Two 16-simulation groups (20 simulations, the second partly filled) give
+    the same trajectories as the one-simulation-per-workgroup FP32 kernel up to
+    FP32 summation order."""
+    rep = np.arange(20) % B
+    c = dict(case, G=case["G"][rep], wEE=case["wEE"][rep], wEI=case["wEI"][rep])
+    kw = dict(n_sims=20)
+    gm = _gpu(c, _cfg(c, bnm.PREC_F32, bnm.KERNEL_MFMA, **kw))
+    gs = _gpu(c, _cfg(c, bnm.PREC_F32, bnm.KERNEL_FAST, **kw))
+    rel = _bold_rel(gm["bold"], gs["bold"])
+    print("mfma vs split FP32 per-TR BOLD rel", rel)
+    np.testing.assert_array_equal(gm["fic_trials"], gs["fic_trials"])
+    assert rel < 1e-3
+    # duplicated parameter sets in different groups integrate identically
+    np.testing.assert_array_equal(gm["bold"][0], gm["bold"][B])
+
+
+@pytest.mark.parametrize("stream_ks", ["1", "2", "4"])
+def test_f64_streamed_sc_kernel_matches_oracle(case, ref, monkeypatch, stream_ks):
+    """The large-N kernel (SC streamed from L2) at small N, forced, against the
+    FP64 oracle."""
+    monkeypatch.setenv("BNM_FORCE_STREAM", "1")
+    monkeypatch.setenv("BNM_STREAM_KS", stream_ks)
+    g = _gpu(case, _cfg(case, bnm.PREC_F64, bnm.KERNEL_FAST))
+    np.testing.assert_array_equal(g["fic_trials"], ref["fic_trials"])
+    rel = _bold_rel(g["bold"], ref["bold"])
+    print("stream KS", stream_ks, "per-TR BOLD rel", rel)
+    assert rel <= 1e-6
+    np.testing.assert_allclose(g["corr"], ref["corr"], atol=1e-9)
+
+
+def test_n400_streamed_f32_vs_f64():
+    """N = 400 (register-resident SC does not fit): FP32 and FP64 streamed
+    kernels agree on the summary statistics over 40 s."""
+    N4, B4, T4 = 400, 4, 40_000
+    SC = W.synthetic_sc(N4, 0.3, 13)
+    rng = np.random.default_rng(4)
+    G = np.array([0.8, 1.5, 2.5, 0.5])
+    wEE = W.heterogeneous(np.array([0.2, 0.25, 0.15, 0.3]), rng.normal(0, 1, N4), 0.02)
+    wEI = W.heterogeneous(np.array([0.15, 0.2, 0.25, 0.1]), rng.normal(0, 1, N4), 0.02)
+    noise = bnm.make_noise_chunked(2, (T4 + 1) * 20 * N4)
+    emp_fc = rng.uniform(-0.2, 0.8, bnm.n_pairs(N4, False))
+    emp_fcd = rng.uniform(-0.5, 1, 40)
+    out = {}
+    for prec in (bnm.PREC_F64, bnm.PREC_F32):
+        cfg = bnm.default_config(nodes=N4, n_sims=B4, time_steps=T4, bold_tr=720, n_vols_remove=0,
+                                 window_size=10, window_step=2, exc_interhemispheric=0,
+                                 noise_period_ms=T4 + 1, precision=prec, kernel=bnm.KERNEL_FAST,
+                                 fc_precision=bnm.FC_F64 if prec == bnm.PREC_F64 else bnm.FC_F32)
+        with bnm.Context(cfg) as c:
+            c.set_sc(SC); c.set_noise(noise); c.set_empirical(emp_fc, emp_fcd)
+            c.set_params(G, wEE, wEI)
+            c.run()
+            print(c.kernel_name(), c.timings())
+            assert c.kernel_name().startswith("stream")
+            out[prec] = c.outputs()
+    a, b = out[bnm.PREC_F32], out[bnm.PREC_F64]
+    np.testing.assert_array_equal(a["fic_trials"], b["fic_trials"])
+    assert np.abs(a["ext"][:, 2] - b["ext"][:, 2]).max() <= 0.05
+    assert np.abs(a["fc"] - b["fc"]).max() <= 1e-3
