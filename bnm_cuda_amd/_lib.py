@@ -31,7 +31,10 @@ EXPORTS = (
     "bnm_window_bounds", "bnm_regional_params", "bnm_make_noise", "bnm_make_noise_chunked", "bnm_run_simulations", "bnm_ctx_create",
     "bnm_ctx_destroy", "bnm_ctx_set_sc", "bnm_ctx_set_noise", "bnm_ctx_set_empirical",
     "bnm_ctx_set_params", "bnm_ctx_run", "bnm_ctx_get_outputs", "bnm_ctx_timings",
-    "bnm_ctx_kernel_name", "bnm_last_error",
+    "bnm_ctx_kernel_name", "bnm_ctx_tune_log", "bnm_multi_create", "bnm_multi_run",
+    "bnm_multi_last_ms", "bnm_multi_destroy", "bnm_cmaes_create", "bnm_cmaes_destroy",
+    "bnm_cmaes_ask", "bnm_cmaes_tell", "bnm_cmaes_best", "bnm_cmaes_sigma", "bnm_cmaes_strategy", "bnm_cmaes_iteration",
+    "bnm_last_error",
 )
 
 
@@ -52,6 +55,7 @@ class Config(C.Structure):
         ("use_fc_diff", C.c_int), ("max_cost", C.c_double), ("fic_penalty_scale", C.c_double),
         ("fic_reject_failed", C.c_int), ("analytical_fic", C.c_int), ("precision", C.c_int),
         ("fc_precision", C.c_int), ("kernel", C.c_int), ("sims_per_block", C.c_int),
+        ("autotune", C.c_int), ("use_graph", C.c_int),
     ]
 
 
@@ -94,6 +98,26 @@ def lib():
     L.bnm_ctx_timings.argtypes = [vp, pd, C.c_int]
     L.bnm_ctx_kernel_name.argtypes = [vp]
     L.bnm_ctx_kernel_name.restype = C.c_char_p
+    L.bnm_ctx_tune_log.argtypes = [vp]
+    L.bnm_ctx_tune_log.restype = C.c_char_p
+    L.bnm_multi_create.argtypes = [C.POINTER(vp), C.c_int, pi]
+    L.bnm_multi_create.restype = vp
+    L.bnm_multi_run.argtypes = [vp]
+    L.bnm_multi_last_ms.argtypes = [vp]
+    L.bnm_multi_last_ms.restype = C.c_double
+    L.bnm_multi_destroy.argtypes = [vp]
+    L.bnm_multi_destroy.restype = None
+    L.bnm_cmaes_create.argtypes = [C.c_int, C.c_int, C.c_uint, C.c_double, C.c_int]
+    L.bnm_cmaes_create.restype = vp
+    L.bnm_cmaes_destroy.argtypes = [vp]
+    L.bnm_cmaes_destroy.restype = None
+    L.bnm_cmaes_ask.argtypes = [vp, pd]
+    L.bnm_cmaes_tell.argtypes = [vp, pd, pd]
+    L.bnm_cmaes_best.argtypes = [vp, pd, pd]
+    L.bnm_cmaes_sigma.argtypes = [vp]
+    L.bnm_cmaes_sigma.restype = C.c_double
+    L.bnm_cmaes_iteration.argtypes = [vp]
+    L.bnm_cmaes_strategy.argtypes = [vp, pd, C.c_int]
     _lib = L
     return L
 
@@ -242,6 +266,9 @@ class Context:
     def kernel_name(self):
         return lib().bnm_ctx_kernel_name(self.h).decode()
 
+    def tune_log(self):
+        return lib().bnm_ctx_tune_log(self.h).decode()
+
     def outputs(self, bold=True, fc=True, fcd=True):
         B, N = self.B, self.N
         o = dict(corr=np.zeros(B), fic_penalty=np.zeros(B), wIE_fic=np.zeros((B, N)),
@@ -260,6 +287,82 @@ class Context:
         o["ext"] = np.ascontiguousarray(np.transpose(e, (1, 0, 2)))  # [B][6][N]
         o["fic_failed"] = o["fic_failed"].astype(bool)
         return o
+
+
+class CMAES:
+    """CMA-ES over the unit box (ask / tell), see bnm_cmaes_* in bnm_hip.h."""
+
+    def __init__(self, dim, lam, seed_mw=1, sigma0=0.5, variant=6):
+        self.dim, self.lam = dim, lam
+        self.h = lib().bnm_cmaes_create(dim, lam, seed_mw, sigma0, variant)
+        if not self.h:
+            raise BnmError(BNM_E_ARG, "bad CMA-ES arguments")
+
+    def ask(self):
+        x = np.zeros((self.lam, self.dim))
+        _chk(lib().bnm_cmaes_ask(self.h, _p(x, C.c_double)))
+        return x
+
+    def tell(self, cost):
+        cost = np.ascontiguousarray(cost, np.float64)
+        pen = np.zeros(self.lam)
+        stopped = _chk(lib().bnm_cmaes_tell(self.h, _p(cost, C.c_double), _p(pen, C.c_double)))
+        return bool(stopped), pen
+
+    def best(self):
+        x = np.zeros(self.dim)
+        c = C.c_double(0)
+        lib().bnm_cmaes_best(self.h, _p(x, C.c_double), C.byref(c))
+        return x, c.value
+
+    @property
+    def sigma(self):
+        return lib().bnm_cmaes_sigma(self.h)
+
+    def strategy(self):
+        out = np.zeros(8 + self.lam)
+        lib().bnm_cmaes_strategy(self.h, _p(out, C.c_double), out.size)
+        keys = ("mueeff", "cc", "csigma", "ccov", "dsigma", "chiN", "c1", "cmue")
+        d = dict(zip(keys, out[:8]))
+        d["weights"] = out[8:]
+        return d
+
+    @property
+    def iteration(self):
+        return lib().bnm_cmaes_iteration(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bnm_cmaes_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Multi:
+    """Several contexts run as parallel branches of one captured hipGraph."""
+
+    def __init__(self, ctxs):
+        self.ctxs = list(ctxs)
+        arr = (C.c_void_p * len(self.ctxs))(*[c.h for c in self.ctxs])
+        st = C.c_int(0)
+        h = lib().bnm_multi_create(arr, len(self.ctxs), C.byref(st))
+        if not h:
+            _chk(st.value if st.value < 0 else BNM_E_HIP)
+        self.h = h
+
+    def run(self):
+        _chk(lib().bnm_multi_run(self.h))
+        return lib().bnm_multi_last_ms(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bnm_multi_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 def run_simulations(cfg: Config, SC, G, wEE, wEI, wIE=None, wIE_fic=None, *, noise,

@@ -57,6 +57,12 @@ int dalloc(T** p, size_t n) {
 
 }  // namespace
 
+// One launchable integration kernel shape.
+struct Variant {
+    int kind;   // 0 split (SC in registers), 1 stream (SC from L2), 2 mfma, 3 plain
+    int ls, ks, mode;
+};
+
 struct bnm_ctx {
     bnm_config cfg{};
     int N = 0, B = 0, ots = 0, np = 0, nw = 0, nseg = 0, m = 0;
@@ -86,6 +92,11 @@ struct bnm_ctx {
     hipEvent_t ev[6] = {};
     double timings[6] = {0, 0, 0, 0, 0, 0};
     std::string kname = "none";
+    std::string tune_log;
+    bool tuned = false;
+    int autotune = 1, tune_ms = 300;
+    Variant variant{0, 4, 1, 0};
+    hipGraphExec_t graph_exec = nullptr;
     bool have_sc = false, have_noise = false, have_emp = false, have_params = false;
     int fast_ls = 0, fast_ks = 0, stream_ks = 0, force_stream = 0;
 };
@@ -124,6 +135,8 @@ int bnm_default_config(bnm_config* c) {
     c->fc_precision = BNM_FC_F32;
     c->kernel = BNM_KERNEL_AUTO;
     c->sims_per_block = 0;
+    c->autotune = env_int("BNM_AUTOTUNE", 1) != 0;
+    c->use_graph = 0;
     return BNM_OK;
 }
 
@@ -244,6 +257,7 @@ void bnm_ctx_destroy(bnm_ctx* x) {
         if (p) (void)hipFree(p);
     for (auto& e : x->ev)
         if (e) (void)hipEventDestroy(e);
+    if (x->graph_exec) (void)hipGraphExecDestroy(x->graph_exec);
     if (x->st) (void)hipStreamDestroy(x->st);
     delete x;
 }
@@ -358,6 +372,8 @@ bnm_ctx* bnm_ctx_create(const bnm_config* cfg, int* status) {
     x->fast_ks = env_int("BNM_FAST_KS", 0);
     x->stream_ks = env_int("BNM_STREAM_KS", 0);
     x->force_stream = env_int("BNM_FORCE_STREAM", 0);
+    x->autotune = c.autotune;
+    x->tune_ms = env_int("BNM_TUNE_MS", 300);
     if (hipStreamCreateWithFlags(&x->st, hipStreamNonBlocking) != hipSuccess) {
         set(fail(BNM_E_HIP, "hipStreamCreate failed"));
         delete x;
@@ -467,74 +483,98 @@ int bnm_ctx_set_params(bnm_ctx* x, const double* G, const double* wEE, const dou
     return BNM_OK;
 }
 
-static int run_integration(bnm_ctx* x, SimArgs& a) {
-    const bnm_config& c = x->cfg;
-    int kern = c.kernel;
-    if (kern == BNM_KERNEL_AUTO) kern = c.precision >= BNM_PREC_BF16 ? BNM_KERNEL_MFMA : BNM_KERNEL_FAST;
-    char name[128];
-    if (kern == BNM_KERNEL_PLAIN) {
-        if (x->f64) HIPCHK(launch_sim_plain<double>(x->SC64, a, x->st));
-        else HIPCHK(launch_sim_plain<float>(x->SC32, a, x->st));
-        std::snprintf(name, sizeof name, "plain_%s(threads=%d)", x->f64 ? "f64" : "f32",
-                      ((x->N + 63) / 64) * 64);
-    } else if (kern == BNM_KERNEL_FAST) {
-        const size_t el = x->f64 ? 8 : 4;
-        int ls = x->fast_ls > 0 ? x->fast_ls : 4, ks = x->fast_ks > 0 ? x->fast_ks : 1;
-        FastShape s = fast_shape(x->N, ls, ks, el);
-        if (x->force_stream) s.ok = false;
-        // otherwise the first register-resident shape in order of preference
-        static const int pref[][2] = {{4, 1}, {2, 1}, {1, 1}, {2, 2}, {1, 2}, {4, 2},
-                                      {1, 4}, {2, 4}, {1, 8}, {2, 8}, {1, 16}};
-        for (const auto& p : pref) {
-            if (s.ok || x->force_stream) break;
-            s = fast_shape(x->N, p[0], p[1], el);
-            if (s.ok) { ls = p[0]; ks = p[1]; }
-        }
-        if (s.ok) {
-            if (x->f64) HIPCHK(launch_sim_fast<double>(x->SC64, a, ls, ks, x->st));
-            else HIPCHK(launch_sim_fast<float>(x->SC32, a, ls, ks, x->st));
-            std::snprintf(name, sizeof name, "split_%s(LS=%d,KS=%d,NW=%d,KC=%d,SS=%d,threads=%d)",
-                          x->f64 ? "f64" : "f32", s.LS, s.KS, s.NW, s.KC, s.SS, s.threads);
-        } else {
-            // SC does not fit in registers: stream it from L2, as many k-slices
-            // as the workgroup size allows
-            int sks = x->stream_ks;
-            if (sks <= 0)
-                for (sks = 16; sks > 1 && !stream_shape(x->N, sks, el).ok; sks /= 2) {}
-            const FastShape g = stream_shape(x->N, sks, el);
-            if (!g.ok) return fail(BNM_E_UNSUP, "no integration shape fits this node count");
-            if (x->f64) HIPCHK(launch_sim_stream<double>(x->SCt64, a, sks, x->st));
-            else HIPCHK(launch_sim_stream<float>(x->SCt32, a, sks, x->st));
-            std::snprintf(name, sizeof name, "stream_%s(KS=%d,NW=%d,kc=%d,threads=%d)",
-                          x->f64 ? "f64" : "f32", g.KS, g.NW, g.kc, g.threads);
-        }
-    } else if (kern == BNM_KERNEL_MFMA) {
-        if (x->f64) return fail(BNM_E_UNSUP, "the MFMA kernel integrates in FP32");
-        const int mode = c.precision == BNM_PREC_F32 ? 0 : c.precision == BNM_PREC_BF16 ? 1 : 2;
-        const MfmaShape s = mfma_shape(x->N, mode);
-        if (!s.ok) return fail(BNM_E_UNSUP, "the MFMA kernel needs N <= 128");
-        HIPCHK(launch_sim_mfma(x->SC32, a, mode, x->st));
-        static const char* mn[] = {"f32", "bf16", "bf16x3"};
-        std::snprintf(name, sizeof name, "mfma_%s(sims/block=16,waves=%d,KQ=%d,threads=%d)", mn[mode],
-                      s.waves, s.KQ, s.threads);
-    } else {
-        return fail(BNM_E_UNSUP, "integration kernel not available");
+// ---- integration variants ------------------------------------------------------
+
+static bool variant_ok(const bnm_ctx* x, const Variant& v) {
+    const size_t el = x->f64 ? 8 : 4;
+    switch (v.kind) {
+        case 0: return fast_shape(x->N, v.ls, v.ks, el).ok;
+        case 1: return stream_shape(x->N, v.ks, el).ok;
+        case 2: return !x->f64 && mfma_shape(x->N, v.mode).ok;
+        default: return true;
     }
-    x->kname = name;
-    return BNM_OK;
 }
 
-int bnm_ctx_run(bnm_ctx* x) {
-    if (!x) return fail(BNM_E_ARG, "null context");
-    if (!x->have_sc || !x->have_noise || !x->have_params || (!x->cfg.sim_only && !x->have_emp))
-        return fail(BNM_E_ARG, "SC, noise, parameters and the empirical target must be set first");
+static std::string variant_name(const bnm_ctx* x, const Variant& v) {
+    const size_t el = x->f64 ? 8 : 4;
+    const char* p = x->f64 ? "f64" : "f32";
+    char name[160];
+    if (v.kind == 0) {
+        const FastShape s = fast_shape(x->N, v.ls, v.ks, el);
+        std::snprintf(name, sizeof name, "split_%s(LS=%d,KS=%d,NW=%d,KC=%d,SS=%d,threads=%d)", p,
+                      s.LS, s.KS, s.NW, s.KC, s.SS, s.threads);
+    } else if (v.kind == 1) {
+        const FastShape g = stream_shape(x->N, v.ks, el);
+        std::snprintf(name, sizeof name, "stream_%s(KS=%d,NW=%d,kc=%d,threads=%d)", p, g.KS, g.NW,
+                      g.kc, g.threads);
+    } else if (v.kind == 2) {
+        static const char* mn[] = {"f32", "bf16", "bf16x3"};
+        const MfmaShape s = mfma_shape(x->N, v.mode);
+        std::snprintf(name, sizeof name, "mfma_%s(sims/block=16,waves=%d,KQ=%d,threads=%d)",
+                      mn[v.mode], s.waves, s.KQ, s.threads);
+    } else {
+        std::snprintf(name, sizeof name, "plain_%s(threads=%d)", p, ((x->N + 63) / 64) * 64);
+    }
+    return name;
+}
+
+static hipError_t launch_variant(bnm_ctx* x, const Variant& v, const SimArgs& a, hipStream_t st) {
+    switch (v.kind) {
+        case 0:
+            return x->f64 ? launch_sim_fast<double>(x->SC64, a, v.ls, v.ks, st)
+                          : launch_sim_fast<float>(x->SC32, a, v.ls, v.ks, st);
+        case 1:
+            return x->f64 ? launch_sim_stream<double>(x->SCt64, a, v.ks, st)
+                          : launch_sim_stream<float>(x->SCt32, a, v.ks, st);
+        case 2: return launch_sim_mfma(x->SC32, a, v.mode, st);
+        default:
+            return x->f64 ? launch_sim_plain<double>(x->SC64, a, st)
+                          : launch_sim_plain<float>(x->SC32, a, st);
+    }
+}
+
+// Candidates that implement the configured precision with the configured kernel.
+static std::vector<Variant> candidates(const bnm_ctx* x) {
     const bnm_config& c = x->cfg;
-    hipStream_t st = x->st;
-    HIPCHK(hipEventRecord(x->ev[0], st));
-    if (c.do_fic && c.analytical_fic)
-        HIPCHK(launch_fic_analytical(x->SC64, x->N, x->B, x->G, x->wEE, x->wEI, x->wIEfic,
-                                     x->unstable, st));
-    HIPCHK(hipEventRecord(x->ev[1], st));
+    std::vector<Variant> out;
+    const int mode = c.precision == BNM_PREC_BF16 ? 1 : c.precision == BNM_PREC_BF16X3 ? 2 : 0;
+    auto add = [&](Variant v) { if (variant_ok(x, v)) out.push_back(v); };
+    if (c.kernel == BNM_KERNEL_PLAIN) {
+        add({3, 0, 0, 0});
+        return out;
+    }
+    if (c.precision >= BNM_PREC_BF16) {
+        add({2, 0, 0, mode});
+        return out;
+    }
+    if (c.kernel == BNM_KERNEL_MFMA) {
+        add({2, 0, 0, 0});
+        return out;
+    }
+    const bool pinned = x->fast_ls > 0 || x->fast_ks > 0;
+    if (pinned && !x->force_stream) {
+        add({0, x->fast_ls > 0 ? x->fast_ls : 4, x->fast_ks > 0 ? x->fast_ks : 1, 0});
+        if (!out.empty()) return out;
+    }
+    if (!x->force_stream) {
+        static const int pref[][2] = {{4, 1}, {2, 1}, {1, 1}, {2, 2}, {1, 2}, {4, 2},
+                                      {1, 4}, {2, 4}, {1, 8}, {2, 8}, {1, 16}};
+        for (const auto& p : pref) add({0, p[0], p[1], 0});
+    }
+    if (out.empty() || x->force_stream) {
+        if (x->stream_ks > 0) {
+            add({1, 1, x->stream_ks, 0});
+        } else {
+            for (int k : {16, 8, 4, 2, 1}) add({1, 1, k, 0});
+        }
+    }
+    // the MFMA kernel implements FP32 too; the tuner may pick it for large populations
+    if (c.kernel == BNM_KERNEL_AUTO && !x->f64 && !pinned && !x->force_stream) add({2, 0, 0, 0});
+    return out;
+}
+
+static SimArgs make_args(bnm_ctx* x) {
+    const bnm_config& c = x->cfg;
     SimArgs a{};
     a.N = x->N;
     a.B = x->B;
@@ -558,9 +598,60 @@ int bnm_ctx_run(bnm_ctx* x) {
     a.ext = x->ext;
     a.fic_failed = x->failed;
     a.fic_trials = x->trials;
-    int r = run_integration(x, a);
-    if (r) return r;
-    HIPCHK(hipEventRecord(x->ev[2], st));
+    return a;
+}
+
+// Picks the integration variant once per context.  With kernel == AUTO and
+// more than one candidate, every candidate integrates the real population for
+// a short window (FIC off) and the fastest by HIP-event time is kept; the
+// window covers the time-loop steady state, where the workgroups-per-CU and
+// per-substep latency trade-off between the variants shows.
+static int choose_variant(bnm_ctx* x) {
+    if (x->tuned) return BNM_OK;
+    std::vector<Variant> cs = candidates(x);
+    if (cs.empty()) return fail(BNM_E_UNSUP, "no integration kernel fits this configuration");
+    x->variant = cs[0];
+    x->tune_log.clear();
+    if (cs.size() > 1 && x->cfg.kernel == BNM_KERNEL_AUTO && x->autotune) {
+        SimArgs a = make_args(x);
+        a.time_steps = std::min(x->cfg.time_steps, x->tune_ms);
+        a.adjust_fic = 0;
+        a.wIE_out = x->wIEscratch;
+        double best = 1e30;
+        for (const Variant& v : cs) {
+            float ms = 0, mn = 1e30f;
+            for (int rep = 0; rep < 2; ++rep) {
+                HIPCHK(hipEventRecord(x->ev[0], x->st));
+                HIPCHK(launch_variant(x, v, a, x->st));
+                HIPCHK(hipEventRecord(x->ev[1], x->st));
+                HIPCHK(hipEventSynchronize(x->ev[1]));
+                HIPCHK(hipEventElapsedTime(&ms, x->ev[0], x->ev[1]));
+                mn = std::min(mn, ms);
+            }
+            char buf[224];
+            std::snprintf(buf, sizeof buf, "%s%s=%.3fms", x->tune_log.empty() ? "" : "; ",
+                          variant_name(x, v).c_str(), mn);
+            x->tune_log += buf;
+            if (mn < best) { best = mn; x->variant = v; }
+        }
+    }
+    x->kname = variant_name(x, x->variant);
+    x->tuned = true;
+    return BNM_OK;
+}
+
+// Enqueues one whole run on stream `st`; with `timed` the phase events are
+// recorded (not inside graph capture).
+static int enqueue_run(bnm_ctx* x, hipStream_t st, bool timed) {
+    const bnm_config& c = x->cfg;
+    if (timed) HIPCHK(hipEventRecord(x->ev[0], st));
+    if (c.do_fic && c.analytical_fic)
+        HIPCHK(launch_fic_analytical(x->SC64, x->N, x->B, x->G, x->wEE, x->wEI, x->wIEfic,
+                                     x->unstable, st));
+    if (timed) HIPCHK(hipEventRecord(x->ev[1], st));
+    const SimArgs a = make_args(x);
+    HIPCHK(launch_variant(x, x->variant, a, st));
+    if (timed) HIPCHK(hipEventRecord(x->ev[2], st));
     if (!c.sim_only) {
         if (x->f64) {
             HIPCHK(launch_seg_moments<double>((const double*)x->bold, x->B, x->N, x->ots, x->nseg,
@@ -586,7 +677,7 @@ int bnm_ctx_run(bnm_ctx* x) {
                     x->mean, x->inv, x->pidx, x->np, (float*)x->fc, (float*)x->wfc, st)));
         }
     }
-    HIPCHK(hipEventRecord(x->ev[3], st));
+    if (timed) HIPCHK(hipEventRecord(x->ev[3], st));
     GofArgs g{};
     g.B = x->B;
     g.N = x->N;
@@ -624,7 +715,7 @@ int bnm_ctx_run(bnm_ctx* x) {
                                                          x->wmean, x->winv, (float*)x->fcd, st)));
         }
     }
-    HIPCHK(hipEventRecord(x->ev[4], st));
+    if (timed) HIPCHK(hipEventRecord(x->ev[4], st));
     if (!c.sim_only && !c.no_fcd) {
         HIPCHK(launch_fill_i32(x->bins, (long long)x->B * (2 * x->m + 1), 0, st));
         HIPCHK(launch_fill_i32(x->nanf, x->B, 0, st));
@@ -633,8 +724,45 @@ int bnm_ctx_run(bnm_ctx* x) {
     }
     if (x->fc64) HIPCHK(launch_gof<double>((const double*)x->fc, g, st));
     else HIPCHK(launch_gof<float>((const float*)x->fc, g, st));
-    HIPCHK(hipEventRecord(x->ev[5], st));
-    HIPCHK(hipStreamSynchronize(st));
+    if (timed) HIPCHK(hipEventRecord(x->ev[5], st));
+    return BNM_OK;
+}
+
+static bool ready(const bnm_ctx* x) {
+    return x->have_sc && x->have_noise && x->have_params && (x->cfg.sim_only || x->have_emp);
+}
+
+int bnm_ctx_run(bnm_ctx* x) {
+    if (!x) return fail(BNM_E_ARG, "null context");
+    if (!ready(x))
+        return fail(BNM_E_ARG, "SC, noise, parameters and the empirical target must be set first");
+    int r = choose_variant(x);
+    if (r) return r;
+    if (x->cfg.use_graph) {
+        if (!x->graph_exec) {
+            hipGraph_t gph = nullptr;
+            HIPCHK(hipStreamBeginCapture(x->st, hipStreamCaptureModeThreadLocal));
+            r = enqueue_run(x, x->st, false);
+            hipError_t e = hipStreamEndCapture(x->st, &gph);
+            if (r) { if (gph) (void)hipGraphDestroy(gph); return r; }
+            HIPCHK(e);
+            e = hipGraphInstantiate(&x->graph_exec, gph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gph);
+            HIPCHK(e);
+        }
+        HIPCHK(hipEventRecord(x->ev[0], x->st));
+        HIPCHK(hipGraphLaunch(x->graph_exec, x->st));
+        HIPCHK(hipEventRecord(x->ev[5], x->st));
+        HIPCHK(hipStreamSynchronize(x->st));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, x->ev[0], x->ev[5]));
+        for (int i = 0; i < 5; ++i) x->timings[i] = -1;
+        x->timings[5] = ms;
+        return BNM_OK;
+    }
+    r = enqueue_run(x, x->st, true);
+    if (r) return r;
+    HIPCHK(hipStreamSynchronize(x->st));
     float ms = 0;
     for (int i = 0; i < 4; ++i) {
         HIPCHK(hipEventElapsedTime(&ms, x->ev[i], x->ev[i + 1]));
@@ -646,6 +774,90 @@ int bnm_ctx_run(bnm_ctx* x) {
     x->timings[5] = ms;
     return BNM_OK;
 }
+
+// ---- several contexts in one graph launch ----------------------------------------
+struct bnm_multi {
+    std::vector<bnm_ctx*> ctxs;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t fork = nullptr, t0 = nullptr, t1 = nullptr;
+    std::vector<hipEvent_t> joins;
+    double ms = 0;
+};
+
+bnm_multi* bnm_multi_create(bnm_ctx** ctxs, int n, int* status) {
+    auto set = [&](int s) { if (status) *status = s; };
+    if (!ctxs || n < 1) { set(fail(BNM_E_ARG, "no contexts")); return nullptr; }
+    for (int i = 0; i < n; ++i)
+        if (!ctxs[i] || !ready(ctxs[i])) {
+            set(fail(BNM_E_ARG, "every context needs SC, noise, parameters and target set"));
+            return nullptr;
+        }
+    auto* m = new bnm_multi;
+    m->ctxs.assign(ctxs, ctxs + n);
+    auto bad = [&](int code) { bnm_multi_destroy(m); set(code); return (bnm_multi*)nullptr; };
+    for (bnm_ctx* x : m->ctxs) {
+        const int r = choose_variant(x);
+        if (r) return bad(r);
+    }
+    if (hipEventCreateWithFlags(&m->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreate(&m->t0) != hipSuccess || hipEventCreate(&m->t1) != hipSuccess)
+        return bad(fail(BNM_E_HIP, "hipEventCreate failed"));
+    m->joins.resize(n, nullptr);
+    for (auto& e : m->joins)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+            return bad(fail(BNM_E_HIP, "hipEventCreate failed"));
+    // capture: the first context's stream forks to every other context's
+    // stream, each enqueues its run, and all join back
+    hipStream_t s0 = m->ctxs[0]->st;
+    hipGraph_t gph = nullptr;
+    if (hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        return bad(fail(BNM_E_HIP, "hipStreamBeginCapture failed"));
+    int r = BNM_OK;
+    hipError_t e = hipEventRecord(m->fork, s0);
+    for (int i = 1; i < n && e == hipSuccess; ++i) e = hipStreamWaitEvent(m->ctxs[i]->st, m->fork, 0);
+    for (int i = 0; i < n && e == hipSuccess && !r; ++i) r = enqueue_run(m->ctxs[i], m->ctxs[i]->st, false);
+    for (int i = 1; i < n && e == hipSuccess && !r; ++i) {
+        e = hipEventRecord(m->joins[i], m->ctxs[i]->st);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s0, m->joins[i], 0);
+    }
+    const hipError_t e2 = hipStreamEndCapture(s0, &gph);
+    if (r || e != hipSuccess || e2 != hipSuccess) {
+        if (gph) (void)hipGraphDestroy(gph);
+        return bad(r ? r : fail(BNM_E_HIP, "graph capture failed"));
+    }
+    e = hipGraphInstantiate(&m->exec, gph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gph);
+    if (e != hipSuccess) return bad(fail(BNM_E_HIP, "hipGraphInstantiate failed"));
+    set(BNM_OK);
+    return m;
+}
+
+int bnm_multi_run(bnm_multi* m) {
+    if (!m) return fail(BNM_E_ARG, "null handle");
+    hipStream_t s0 = m->ctxs[0]->st;
+    HIPCHK(hipEventRecord(m->t0, s0));
+    HIPCHK(hipGraphLaunch(m->exec, s0));
+    HIPCHK(hipEventRecord(m->t1, s0));
+    HIPCHK(hipStreamSynchronize(s0));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, m->t0, m->t1));
+    m->ms = ms;
+    return BNM_OK;
+}
+
+double bnm_multi_last_ms(bnm_multi* m) { return m ? m->ms : -1; }
+
+void bnm_multi_destroy(bnm_multi* m) {
+    if (!m) return;
+    if (m->exec) (void)hipGraphExecDestroy(m->exec);
+    for (hipEvent_t e : {m->fork, m->t0, m->t1})
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : m->joins)
+        if (e) (void)hipEventDestroy(e);
+    delete m;
+}
+
+const char* bnm_ctx_tune_log(bnm_ctx* x) { return x ? x->tune_log.c_str() : ""; }
 
 int bnm_ctx_get_outputs(bnm_ctx* x, double* corr, double* pen, double* wIE_fic, double* BOLD,
                         double* S_E, double* I_E, double* r_E, double* S_I, double* I_I,

@@ -86,7 +86,13 @@ typedef struct bnm_config {
     int precision;             /* BNM_PREC_*                                      */
     int fc_precision;          /* BNM_FC_*                                        */
     int kernel;                /* BNM_KERNEL_*                                    */
-    int sims_per_block;        /* MFMA kernel: simulations per workgroup (0=auto) */
+    int sims_per_block;        /* reserved (the MFMA kernel groups 16)            */
+    int autotune;              /* kernel == AUTO: time the candidate kernels on a
+                                  short window at the first run and keep the
+                                  fastest (env BNM_AUTOTUNE)                      */
+    int use_graph;             /* capture the whole run in a hipGraph at the first
+                                  run and replay it (per-phase timings are then
+                                  not available, only the total)                 */
 } bnm_config;
 
 /* Fills `cfg` with the reference defaults (constants.h:93-116), honouring the
@@ -182,6 +188,40 @@ int bnm_ctx_timings(bnm_ctx* ctx, double* ms, int n);
 /* Name of the integration kernel variant the last run used, and its launch
  * shape (for reports). */
 const char* bnm_ctx_kernel_name(bnm_ctx* ctx);
+
+/* Integration variants the autotuner timed and their times (empty when it
+ * did not run). */
+const char* bnm_ctx_tune_log(bnm_ctx* ctx);
+
+/* ---- several populations in one graph launch: each context (its own SC,
+ * node count, duration and parameters) becomes a parallel branch of one
+ * captured hipGraph; bnm_multi_run launches it once and waits.  Contexts
+ * must have their inputs set; outputs are read per context afterwards.
+ * No counterpart in the reference (one population per call). ---- */
+typedef struct bnm_multi bnm_multi;
+bnm_multi* bnm_multi_create(bnm_ctx** ctxs, int n, int* status);
+int bnm_multi_run(bnm_multi* m);
+double bnm_multi_last_ms(bnm_multi* m);
+void bnm_multi_destroy(bnm_multi* m);
+
+/* ---- CMA-ES over the unit box (cmaes.cpp of the reference: CMAES_Points,
+ * Hansen 2006 / 2016 variants, IQR boundary penalty, early stop).  Caller maps
+ * the unit box onto parameter ranges (run_CMAES.cpp:727-742). ---- */
+typedef struct bnm_cmaes bnm_cmaes;
+/* dim free parameters, lambda candidates, seed of the initial mean (SeedMW),
+ * initial sigma (BNM_CMAES_SIGMA, 0.5), variant 6 (2006) or 16 (2016). */
+bnm_cmaes* bnm_cmaes_create(int dim, int lambda, unsigned seed_mw, double sigma0, int variant);
+void bnm_cmaes_destroy(bnm_cmaes* e);
+/* current candidates clipped to the unit box, x[j*dim + i] */
+int bnm_cmaes_ask(bnm_cmaes* e, double* x_feas);
+/* costs of the current candidates (lower is better); penalized (optional)
+ * receives cost + boundary penalty.  Returns 1 when the run stopped. */
+int bnm_cmaes_tell(bnm_cmaes* e, const double* cost, double* penalized);
+int bnm_cmaes_best(bnm_cmaes* e, double* x, double* raw_cost);
+double bnm_cmaes_sigma(bnm_cmaes* e);
+/* [mueeff, cc, csigma, ccov, dsigma, chiN, c1, cmue, weights[lambda]] */
+int bnm_cmaes_strategy(bnm_cmaes* e, double* out, int n);
+int bnm_cmaes_iteration(bnm_cmaes* e);
 
 /* Last error message of this thread (static storage). */
 const char* bnm_last_error(void);

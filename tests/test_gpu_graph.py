@@ -1,0 +1,107 @@
+"""Autotuned kernel choice, hipGraph replay of a whole run, and several
+populations (different node counts and durations) as parallel branches of one
+graph launch.  Every graph replay is compared bitwise with an eager run of the
+same inputs; replays are interleaved with eager runs and host syncs."""
+import numpy as np
+import pytest
+
+import bnm_cuda_amd as bnm
+from bnm_cuda_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("corr", "fic_penalty", "bold", "fc", "fcd", "fic_trials", "wIE_fic", "ext")
+
+
+def _population(N, B, T, seed, precision=bnm.PREC_F64, **kw):
+    rng = np.random.default_rng(seed)
+    SC = W.synthetic_sc(N, 0.3, seed)
+    G = rng.uniform(0.3, 2.5, B)
+    wEE = W.heterogeneous(rng.uniform(0.1, 0.3, B), rng.normal(0, 1, N), 0.02)
+    wEI = W.heterogeneous(rng.uniform(0.1, 0.3, B), rng.normal(0, 1, N), 0.02)
+    noise = bnm.make_noise(seed, (T + 1) * 20 * N)
+    cfg = bnm.default_config(nodes=N, n_sims=B, time_steps=T, bold_tr=1000, n_vols_remove=10,
+                             window_size=10, window_step=2, noise_period_ms=T + 1, max_fic_trials=3,
+                             precision=precision,
+                             fc_precision=bnm.FC_F64 if precision == bnm.PREC_F64 else bnm.FC_F32,
+                             **kw)
+    np_ = bnm.n_pairs(N, True)
+    emp = (rng.uniform(-0.2, 0.8, np_), rng.uniform(-0.5, 1, 60))
+    return cfg, SC, (G, wEE, wEI), noise, emp
+
+
+def _ctx(cfg, SC, params, noise, emp):
+    c = bnm.Context(cfg)
+    c.set_sc(SC)
+    c.set_noise(noise)
+    c.set_empirical(*emp)
+    c.set_params(*params)
+    return c
+
+
+def _same(a, b):
+    for k in KEYS:
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
+def test_autotuner_picks_a_timed_variant():
+    print("case: autotune N=20 B=6 T=60 s f32", flush=True)
+    cfg, SC, params, noise, emp = _population(20, 6, 60_000, 1, bnm.PREC_F32)
+    with _ctx(cfg, SC, params, noise, emp) as c:
+        c.run()
+        log = c.tune_log()
+        print("tuned:", c.kernel_name(), "|", log)
+        assert log.count("ms") >= 2 and c.kernel_name() in log
+        o = c.outputs()
+    cfg_fixed = bnm.Config.from_buffer_copy(cfg)
+    cfg_fixed.kernel = bnm.KERNEL_FAST
+    with _ctx(cfg_fixed, SC, params, noise, emp) as c2:
+        c2.run()
+        o2 = c2.outputs()
+    # every candidate implements the same model; FP32 trajectories agree closely
+    np.testing.assert_array_equal(o["fic_trials"], o2["fic_trials"])
+    assert np.abs(o["fc"] - o2["fc"]).max() < 1e-3
+
+
+@pytest.mark.parametrize("precision", [bnm.PREC_F64, bnm.PREC_F32])
+def test_graph_replay_matches_eager(precision):
+    print("case: graph replay N=20 B=6 T=60 s precision", precision, flush=True)
+    cfg, SC, params, noise, emp = _population(20, 6, 60_000, 2, precision, autotune=0)
+    eager = _ctx(cfg, SC, params, noise, emp)
+    eager.run()
+    ref = eager.outputs()
+    gcfg = bnm.Config.from_buffer_copy(cfg)
+    gcfg.use_graph = 1
+    gctx = _ctx(gcfg, SC, params, noise, emp)
+    for rep in range(3):
+        gctx.run()
+        _same(gctx.outputs(), ref)
+        eager.run()                      # eager work between the replays
+        _same(eager.outputs(), ref)
+    assert gctx.timings()["total"] > 0
+    gctx.close()
+    eager.close()
+
+
+def test_uneven_populations_in_one_graph_launch():
+    """Three populations with N = 20, 34, 68 and durations 60, 80, 70 s run as
+    branches of one graph; each matches its own eager run bitwise."""
+    specs = [(20, 6, 60_000, 3), (34, 5, 80_000, 4), (68, 17, 70_000, 5)]
+    ctxs, refs = [], []
+    for N, B, T, seed in specs:
+        print("case: multi branch N", N, "B", B, "T", T, flush=True)
+        pop = _population(N, B, T, seed, bnm.PREC_F32, autotune=0)
+        e = _ctx(*pop)
+        e.run()
+        refs.append(e.outputs())
+        e.close()
+        ctxs.append(_ctx(*pop))
+    m = bnm.Multi(ctxs)
+    for rep in range(2):
+        ms = m.run()
+        assert ms > 0
+        for c, r in zip(ctxs, refs):
+            _same(c.outputs(), r)
+    m.close()
+    for c in ctxs:
+        c.close()
