@@ -34,6 +34,9 @@ EXPORTS = (
     "bnm_ctx_kernel_name", "bnm_ctx_tune_log", "bnm_multi_create", "bnm_multi_run",
     "bnm_multi_last_ms", "bnm_multi_destroy", "bnm_cmaes_create", "bnm_cmaes_destroy",
     "bnm_cmaes_ask", "bnm_cmaes_tell", "bnm_cmaes_best", "bnm_cmaes_sigma", "bnm_cmaes_strategy", "bnm_cmaes_iteration",
+    "bnm_mlp_create", "bnm_mlp_destroy", "bnm_mlp_num_params", "bnm_mlp_get_params",
+    "bnm_mlp_set_params", "bnm_mlp_predict", "bnm_mlp_loss_grad", "bnm_mlp_set_data",
+    "bnm_mlp_train", "bnm_mlp_last_error",
     "bnm_last_error",
 )
 
@@ -118,6 +121,19 @@ def lib():
     L.bnm_cmaes_sigma.restype = C.c_double
     L.bnm_cmaes_iteration.argtypes = [vp]
     L.bnm_cmaes_strategy.argtypes = [vp, pd, C.c_int]
+    L.bnm_mlp_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint, pi]
+    L.bnm_mlp_create.restype = vp
+    L.bnm_mlp_destroy.argtypes = [vp]
+    L.bnm_mlp_destroy.restype = None
+    L.bnm_mlp_num_params.argtypes = [vp]
+    L.bnm_mlp_num_params.restype = C.c_long
+    L.bnm_mlp_get_params.argtypes = [vp, pf]
+    L.bnm_mlp_set_params.argtypes = [vp, pf]
+    L.bnm_mlp_predict.argtypes = [vp, pf, C.c_int, pf]
+    L.bnm_mlp_loss_grad.argtypes = [vp, pf, pf, C.c_int, pf, pf]
+    L.bnm_mlp_set_data.argtypes = [vp, pf, pf, C.c_int]
+    L.bnm_mlp_train.argtypes = [vp, C.c_int, C.c_int, C.c_float, C.c_uint, pf]
+    L.bnm_mlp_last_error.restype = C.c_char_p
     _lib = L
     return L
 
@@ -335,6 +351,86 @@ class CMAES:
         if getattr(self, "h", None):
             lib().bnm_cmaes_destroy(self.h)
             self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def _chk_mlp(code):
+    if code < 0:
+        raise BnmError(code, lib().bnm_mlp_last_error().decode())
+    return code
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, np.float32)
+    return a if shape is None else a.reshape(shape)
+
+
+class MLP:
+    """Dense GELU network trained on the device (bnm_mlp_* in bnm_hip.h).
+
+    Layers in -> hidden x depth -> out; the flat parameter vector holds W_l
+    [out_l x in_l] row-major then b_l per layer."""
+
+    def __init__(self, in_dim, hidden=256, depth=4, out_dim=1, bf16=True, seed=0):
+        st = C.c_int(0)
+        self.in_dim, self.hidden, self.depth, self.out_dim = in_dim, hidden, depth, out_dim
+        self.h = lib().bnm_mlp_create(in_dim, hidden, depth, out_dim, int(bool(bf16)), seed, C.byref(st))
+        if not self.h:
+            _chk_mlp(st.value if st.value < 0 else BNM_E_ARG)
+        self.n_params = lib().bnm_mlp_num_params(self.h)
+
+    def dims(self):
+        return [self.in_dim] + [self.hidden] * self.depth + [self.out_dim]
+
+    def get_params(self):
+        p = np.zeros(self.n_params, np.float32)
+        _chk_mlp(lib().bnm_mlp_get_params(self.h, _p(p, C.c_float)))
+        return p
+
+    def set_params(self, p):
+        p = _f32(p)
+        assert p.size == self.n_params
+        _chk_mlp(lib().bnm_mlp_set_params(self.h, _p(p, C.c_float)))
+
+    def predict(self, X):
+        X = _f32(X)
+        n = X.shape[0]
+        assert X.shape[1] == self.in_dim
+        Y = np.zeros((n, self.out_dim), np.float32)
+        _chk_mlp(lib().bnm_mlp_predict(self.h, _p(X, C.c_float), n, _p(Y, C.c_float)))
+        return Y
+
+    def loss_grad(self, X, T):
+        X, T = _f32(X), _f32(T)
+        n = X.shape[0]
+        assert X.shape == (n, self.in_dim) and T.shape == (n, self.out_dim)
+        loss = C.c_float(0)
+        g = np.zeros(self.n_params, np.float32)
+        _chk_mlp(lib().bnm_mlp_loss_grad(self.h, _p(X, C.c_float), _p(T, C.c_float), n, C.byref(loss),
+                                         _p(g, C.c_float)))
+        return loss.value, g
+
+    def fit(self, X, T, epochs=10, batch=1024, lr=1e-3, seed=0):
+        X, T = _f32(X), _f32(T)
+        n = X.shape[0]
+        assert X.shape == (n, self.in_dim) and T.shape == (n, self.out_dim)
+        _chk_mlp(lib().bnm_mlp_set_data(self.h, _p(X, C.c_float), _p(T, C.c_float), n))
+        losses = np.zeros(epochs, np.float32)
+        _chk_mlp(lib().bnm_mlp_train(self.h, epochs, batch, lr, seed, _p(losses, C.c_float)))
+        return losses
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bnm_mlp_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
     def __del__(self):
         self.close()

@@ -223,6 +223,29 @@ double bnm_cmaes_sigma(bnm_cmaes* e);
 int bnm_cmaes_strategy(bnm_cmaes* e, double* out, int n);
 int bnm_cmaes_iteration(bnm_cmaes* e);
 
+/* ---- small MLP trained on the device (dense layers, GELU, mean squared
+ * error, Adam), for the learned FIC initialiser and the fit surrogate.  No
+ * counterpart in the reference.  Layers: in -> hidden x depth -> out; the
+ * flat parameter vector holds, per layer l, W_l [out_l x in_l] row-major then
+ * b_l [out_l].  bf16 != 0: GEMM operands rounded to bfloat16 (FP32
+ * accumulation, FP32 parameters and Adam state); 0: FP32 operands. ---- */
+typedef struct bnm_mlp bnm_mlp;
+bnm_mlp* bnm_mlp_create(int in_dim, int hidden, int depth, int out_dim, int bf16, unsigned seed,
+                        int* status);
+void bnm_mlp_destroy(bnm_mlp* m);
+long bnm_mlp_num_params(bnm_mlp* m);
+int bnm_mlp_get_params(bnm_mlp* m, float* out);
+int bnm_mlp_set_params(bnm_mlp* m, const float* in);
+/* Y[n x out] from X[n x in] (host arrays) */
+int bnm_mlp_predict(bnm_mlp* m, const float* X, int n, float* Y);
+/* mean squared error over n x out and its gradient (flat, host) */
+int bnm_mlp_loss_grad(bnm_mlp* m, const float* X, const float* T, int n, float* loss, float* grad);
+/* upload a training set, then Adam over shuffled mini-batches; loss[e] is
+ * the mean batch loss of epoch e (optional) */
+int bnm_mlp_set_data(bnm_mlp* m, const float* X, const float* T, int n);
+int bnm_mlp_train(bnm_mlp* m, int epochs, int batch, float lr, unsigned seed, float* loss);
+const char* bnm_mlp_last_error(void);
+
 /* Last error message of this thread (static storage). */
 const char* bnm_last_error(void);
 
