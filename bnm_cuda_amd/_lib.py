@@ -33,7 +33,7 @@ EXPORTS = (
     "bnm_ctx_set_params", "bnm_ctx_run", "bnm_ctx_get_outputs", "bnm_ctx_timings",
     "bnm_ctx_kernel_name", "bnm_ctx_tune_log", "bnm_multi_create", "bnm_multi_run",
     "bnm_multi_last_ms", "bnm_multi_destroy", "bnm_cmaes_create", "bnm_cmaes_destroy",
-    "bnm_cmaes_ask", "bnm_cmaes_tell", "bnm_cmaes_best", "bnm_cmaes_sigma", "bnm_cmaes_strategy", "bnm_cmaes_iteration",
+    "bnm_cmaes_ask", "bnm_cmaes_tell", "bnm_cmaes_best", "bnm_cmaes_sigma", "bnm_cmaes_strategy", "bnm_cmaes_iteration", "bnm_cmaes_ask_pool", "bnm_cmaes_select",
     "bnm_mlp_create", "bnm_mlp_destroy", "bnm_mlp_num_params", "bnm_mlp_get_params",
     "bnm_mlp_set_params", "bnm_mlp_predict", "bnm_mlp_loss_grad", "bnm_mlp_set_data",
     "bnm_mlp_train", "bnm_mlp_last_error",
@@ -121,6 +121,8 @@ def lib():
     L.bnm_cmaes_sigma.restype = C.c_double
     L.bnm_cmaes_iteration.argtypes = [vp]
     L.bnm_cmaes_strategy.argtypes = [vp, pd, C.c_int]
+    L.bnm_cmaes_ask_pool.argtypes = [vp, C.c_int, C.c_uint, pd]
+    L.bnm_cmaes_select.argtypes = [vp, pi]
     L.bnm_mlp_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint, pi]
     L.bnm_mlp_create.restype = vp
     L.bnm_mlp_destroy.argtypes = [vp]
@@ -324,6 +326,17 @@ class CMAES:
         pen = np.zeros(self.lam)
         stopped = _chk(lib().bnm_cmaes_tell(self.h, _p(cost, C.c_double), _p(pen, C.c_double)))
         return bool(stopped), pen
+
+    def ask_pool(self, n, seed):
+        """n >= lam candidates: the current sample, then n - lam more draws."""
+        x = np.zeros((n, self.dim))
+        _chk(lib().bnm_cmaes_ask_pool(self.h, n, seed, _p(x, C.c_double)))
+        return x
+
+    def select(self, idx):
+        idx = np.ascontiguousarray(idx, np.int32)
+        assert idx.size == self.lam
+        _chk(lib().bnm_cmaes_select(self.h, _p(idx, C.c_int)))
 
     def best(self):
         x = np.zeros(self.dim)

@@ -93,6 +93,8 @@ struct bnm_cmaes {
     double cc = 0, ccov = 0, csigma = 0, dsigma = 0, chiN = 0, muecov = 0, mueeff = 0, c1 = 0, cmue = 0;
     std::vector<double> weights, Gamma, ksi, psigma, pc, m, malt, mpref, IQAhist, C, B, Dd;
     std::vector<double> X, Xfeas, Xpref;   // [D][lambda]
+    std::vector<double> pool;              // [n_pool][D], raw (unclipped) points
+    int n_pool = 0;
     std::vector<double> best_x, tvec;
     double best_val = -INFINITY, best_raw = 0;
     int stopped = 0;
@@ -380,6 +382,51 @@ int bnm_cmaes_strategy(bnm_cmaes* e, double* out, int n) {
     const int k = std::min(n, (int)v.size());
     std::copy(v.begin(), v.begin() + k, out);
     return k;
+}
+
+// Pre-selection support.  Draws a pool of n >= lambda candidates from the
+// current distribution N(m, sigma^2 C): the first lambda are the current
+// sample (what bnm_cmaes_ask returns), the other n - lambda come from a
+// generator seeded with `seed`.  x_feas[j*D + i] receives the clipped points.
+int bnm_cmaes_ask_pool(bnm_cmaes* e, int n, unsigned seed, double* x_feas) {
+    if (!e || !x_feas || n < e->lambda) return BNM_E_ARG;
+    const int D = e->D, L = e->lambda;
+    e->pool.assign((size_t)n * D, 0.0);
+    for (int j = 0; j < L; ++j)
+        for (int i = 0; i < D; ++i) e->pool[(size_t)j * D + i] = e->X[(size_t)i * L + j];
+    std::default_random_engine g;
+    g.seed(seed);
+    std::normal_distribution<double> nd(0, 1.0);
+    std::vector<double> t(D);
+    for (int j = L; j < n; ++j) {
+        for (int i = 0; i < D; ++i) t[i] = e->Dd[i] * nd(g);
+        for (int i = 0; i < D; ++i) {
+            double s = 0;
+            for (int k = 0; k < D; ++k) s += e->B[(size_t)i * D + k] * t[k];
+            e->pool[(size_t)j * D + i] = e->m[i] + e->sigma * s;
+        }
+    }
+    for (size_t q = 0; q < e->pool.size(); ++q) x_feas[q] = std::min(1.0, std::max(0.0, e->pool[q]));
+    e->n_pool = n;
+    return BNM_OK;
+}
+
+// Makes pool members idx[0..lambda) the current sample (the points the next
+// bnm_cmaes_tell scores), e.g. the lambda best by a surrogate.
+int bnm_cmaes_select(bnm_cmaes* e, const int* idx) {
+    if (!e || !idx || e->n_pool < e->lambda) return BNM_E_ARG;
+    const int D = e->D, L = e->lambda;
+    for (int j = 0; j < L; ++j)
+        if (idx[j] < 0 || idx[j] >= e->n_pool) return BNM_E_ARG;
+    for (int j = 0; j < L; ++j)
+        for (int i = 0; i < D; ++i) {
+            const double v = e->pool[(size_t)idx[j] * D + i];
+            e->X[(size_t)i * L + j] = v;
+            e->Xfeas[(size_t)i * L + j] = std::min(1.0, std::max(0.0, v));
+            e->Xpref[(size_t)i * L + j] = std::min(1.0 - e->soft_edge, std::max(e->soft_edge, v));
+        }
+    e->n_pool = 0;
+    return BNM_OK;
 }
 
 double bnm_cmaes_sigma(bnm_cmaes* e) { return e ? e->sigma : -1; }

@@ -222,6 +222,13 @@ double bnm_cmaes_sigma(bnm_cmaes* e);
 /* [mueeff, cc, csigma, ccov, dsigma, chiN, c1, cmue, weights[lambda]] */
 int bnm_cmaes_strategy(bnm_cmaes* e, double* out, int n);
 int bnm_cmaes_iteration(bnm_cmaes* e);
+/* Pre-selection (no counterpart in the reference): a pool of n >= lambda
+ * candidates from the current distribution, the first lambda being the
+ * current sample and the rest drawn with `seed`; x_feas[j*dim + i] clipped.
+ * bnm_cmaes_select then makes pool members idx[0..lambda) the sample that the
+ * next bnm_cmaes_tell scores. */
+int bnm_cmaes_ask_pool(bnm_cmaes* e, int n, unsigned seed, double* x_feas);
+int bnm_cmaes_select(bnm_cmaes* e, const int* idx);
 
 /* ---- small MLP trained on the device (dense layers, GELU, mean squared
  * error, Adam), for the learned FIC initialiser and the fit surrogate.  No
