@@ -28,7 +28,7 @@ FC_F64, FC_F32, FC_BF16 = 0, 1, 2
 # every symbol include/bnm_hip.h declares
 EXPORTS = (
     "bnm_default_config", "bnm_finalize_config", "bnm_output_ts", "bnm_n_pairs", "bnm_n_windows",
-    "bnm_window_bounds", "bnm_regional_params", "bnm_make_noise", "bnm_make_noise_chunked", "bnm_run_simulations", "bnm_ctx_create",
+    "bnm_window_bounds", "bnm_regional_params", "bnm_analytical_fic", "bnm_make_noise", "bnm_make_noise_chunked", "bnm_run_simulations", "bnm_ctx_create",
     "bnm_ctx_destroy", "bnm_ctx_set_sc", "bnm_ctx_set_noise", "bnm_ctx_set_empirical",
     "bnm_ctx_set_params", "bnm_ctx_run", "bnm_ctx_get_outputs", "bnm_ctx_timings",
     "bnm_ctx_kernel_name", "bnm_ctx_tune_log", "bnm_multi_create", "bnm_multi_run",
@@ -85,6 +85,7 @@ def lib():
     L.bnm_window_bounds.argtypes = [pc, pi, pi]
     L.bnm_make_noise.argtypes = [C.c_uint, C.c_longlong, pf]
     L.bnm_regional_params.argtypes = [C.c_int, C.c_int, pd, C.c_double, pd, pd]
+    L.bnm_analytical_fic.argtypes = [C.c_int, C.c_int, pd, pd, pd, pd, pd, pi]
     L.bnm_make_noise_chunked.argtypes = [C.c_uint, C.c_longlong, pf, C.c_int]
     L.bnm_run_simulations.argtypes = [pc, pd, pd, pd, pd, pd, pd, pd, pd, pf, C.c_longlong,
                                       pd, C.c_int, pd, C.c_int] + [pd] * 7 + [pi, pi, pd, pd]
@@ -192,6 +193,21 @@ def regional_params(maps, base, scales) -> np.ndarray:
     _chk(lib().bnm_regional_params(N, n_maps, _p(maps, C.c_double), float(base),
                                    _p(scales, C.c_double), _p(out, C.c_double)))
     return out
+
+
+def analytical_fic(SC, G, wEE, wEI):
+    """Analytical FIC wIE [B, N] and unstable flags [B] (device computation)."""
+    SC = np.ascontiguousarray(SC, np.float64)
+    N = SC.shape[0]
+    G = np.ascontiguousarray(np.atleast_1d(G), np.float64)
+    B = G.size
+    wEE = np.ascontiguousarray(np.broadcast_to(wEE, (B, N)), np.float64)
+    wEI = np.ascontiguousarray(np.broadcast_to(wEI, (B, N)), np.float64)
+    out = np.zeros((B, N))
+    un = np.zeros(B, np.int32)
+    _chk(lib().bnm_analytical_fic(N, B, _p(SC, C.c_double), _p(G, C.c_double), _p(wEE, C.c_double),
+                                  _p(wEI, C.c_double), _p(out, C.c_double), _p(un, C.c_int)))
+    return out, un.astype(bool)
 
 
 def make_noise(seed: int, count: int) -> np.ndarray:

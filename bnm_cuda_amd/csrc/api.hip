@@ -325,21 +325,56 @@ static int ctx_alloc(bnm_ctx* x) {
     return BNM_OK;
 }
 
-bnm_ctx* bnm_ctx_create(const bnm_config* cfg, int* status) {
-    auto set = [&](int s) { if (status) *status = s; };
-    if (!cfg) { set(fail(BNM_E_ARG, "null config")); return nullptr; }
+// BNM_OK when the current HIP device is a gfx950, else BNM_E_NODEV
+static int check_device() {
     int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
-        set(fail(BNM_E_NODEV, "no HIP device"));
-        return nullptr;
-    }
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(BNM_E_NODEV, "no HIP device");
     int dev = 0;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess ||
-        std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
-        set(fail(BNM_E_NODEV, "the current HIP device is not gfx950"));
-        return nullptr;
-    }
+        std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(BNM_E_NODEV, "the current HIP device is not gfx950");
+    return BNM_OK;
+}
+
+// Analytical FIC for B parameter sets on the device (the reference's
+// analytical_fic, fic.cpp:130-207, called per particle by calculate_params,
+// run_CMAES.cpp:144-174).
+int bnm_analytical_fic(int nodes, int n_sims, const double* SC, const double* G, const double* wEE,
+                       const double* wEI, double* wIE_out, int* unstable) {
+    if (nodes < 1 || n_sims < 1 || !SC || !G || !wEE || !wEI || !wIE_out)
+        return fail(BNM_E_ARG, "bnm_analytical_fic: bad arguments");
+    int r = check_device();
+    if (r) return r;
+    const size_t N = nodes, B = n_sims;
+    double *dSC = nullptr, *dG = nullptr, *dEE = nullptr, *dEI = nullptr, *dIE = nullptr;
+    int* dU = nullptr;
+    auto run = [&]() -> int {
+        HIPCHK(hipMalloc(&dSC, N * N * sizeof(double)));
+        HIPCHK(hipMalloc(&dG, B * sizeof(double)));
+        HIPCHK(hipMalloc(&dEE, B * N * sizeof(double)));
+        HIPCHK(hipMalloc(&dEI, B * N * sizeof(double)));
+        HIPCHK(hipMalloc(&dIE, B * N * sizeof(double)));
+        HIPCHK(hipMalloc(&dU, B * sizeof(int)));
+        HIPCHK(hipMemcpy(dSC, SC, N * N * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dG, G, B * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dEE, wEE, B * N * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dEI, wEI, B * N * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(launch_fic_analytical(dSC, nodes, n_sims, dG, dEE, dEI, dIE, dU, nullptr));
+        HIPCHK(hipMemcpy(wIE_out, dIE, B * N * sizeof(double), hipMemcpyDeviceToHost));
+        if (unstable) HIPCHK(hipMemcpy(unstable, dU, B * sizeof(int), hipMemcpyDeviceToHost));
+        return BNM_OK;
+    };
+    r = run();
+    for (void* p : {(void*)dSC, (void*)dG, (void*)dEE, (void*)dEI, (void*)dIE, (void*)dU})
+        if (p) (void)hipFree(p);
+    return r;
+}
+
+bnm_ctx* bnm_ctx_create(const bnm_config* cfg, int* status) {
+    auto set = [&](int s) { if (status) *status = s; };
+    if (!cfg) { set(fail(BNM_E_ARG, "null config")); return nullptr; }
+    if (int r = check_device()) { set(r); return nullptr; }
     bnm_config c = *cfg;
     int r = bnm_finalize_config(&c);
     if (r) { set(r); return nullptr; }

@@ -1,0 +1,684 @@
+// run_cmaes.cpp -- CMA-ES fitting of the reduced Wong-Wang network to an
+// empirical FC / FCD on the GPU: the command-line program of the reference's
+// run_CMAES.cpp (GPU build, main at run_CMAES.cpp:176-906) over libbnm_hip.
+//
+// Same 19 positional arguments, input files, BNM_* environment, output
+// folders, log lines and best-simulation text files as the reference:
+//   run_cmaes <sc_path> <out_prefix|'same'> <fc_tril_path> <fcd_tril_path|no_fcd>
+//             <maps_path|no_maps> <n_regions> <G_min[-G_max]> <wee_min[-wee_max]>
+//             <wei_min[-wei_max]> <wie_min[-wie_max]> <het_params> <time_steps>
+//             <BOLD_TR> <fcd_step> <fcd_window> <rand_seed> <lambda> <itMax> <SeedMW>
+//
+// Additions (environment):
+//   BNM_PRECISION      f64 (default, as the reference's double build) | f32 | bf16x3
+//   BNM_NOISE_CHUNKED  1: parallel chunked noise generator (a different stream)
+//   BNM_SURROGATE      1: surrogate-guided search (below)
+//   BNM_PROGRESS_CSV   1 (default): per-generation progress file next to the log
+//
+// Surrogate-guided search (BNM_SURROGATE=1):
+//  * FIC initialiser: before the search, BNM_SURR_FIC_SIMS parameter sets drawn
+//    uniformly from the search box are run through the FIC phase only
+//    (BNM_SURR_FIC_TRIALS restarts allowed); an MLP (inputs G, wEE_j, wEI_j,
+//    SC strength_j, analytical wIE_j; 4 x 256 GELU; BF16 MFMA) learns the
+//    converged wIE_j minus the analytical one.  Every simulation of the search
+//    then starts FIC from analytical + predicted wIE.
+//  * fit surrogate: an MLP on the unit-box coordinates predicts
+//    (FC r, FCD KS, |mean FC difference|, FIC penalty) of a candidate.  It is
+//    trained on every simulation run so far.  From generation
+//    BNM_SURR_WARMUP + 1 on, each generation draws BNM_SURR_POOL x lambda
+//    candidates, ranks them by predicted cost and simulates the lambda best in
+//    full; when the rank correlation of predicted and simulated cost of the
+//    previous generation falls below BNM_SURR_MIN_RHO, the next generation is
+//    the plain CMA-ES sample (fallback).
+//  * every reported cost is a full simulation; the best point is re-run with
+//    the plain method (analytical FIC init) at the end, as in the reference.
+#include <sys/stat.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <numeric>
+#include <random>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../../include/bnm_hip.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double env_d(const char* k, double d) {
+    const char* v = std::getenv(k);
+    return v ? std::atof(v) : d;
+}
+int env_i(const char* k, int d) {
+    const char* v = std::getenv(k);
+    return v ? std::atoi(v) : d;
+}
+std::string env_s(const char* k, const char* d) {
+    const char* v = std::getenv(k);
+    return v ? std::string(v) : std::string(d);
+}
+
+[[noreturn]] void die(const std::string& m) {
+    std::fprintf(stderr, "%s\n", m.c_str());
+    std::exit(1);
+}
+
+void check(int st, const char* what) {
+    if (st < 0) die(std::string(what) + " failed: " + bnm_last_error());
+}
+
+std::vector<double> read_values(const std::string& path, const char* what) {
+    std::ifstream f(path);
+    if (!f) die(std::string("Error opening ") + what + " file");
+    std::vector<double> v;
+    double x;
+    while (f >> x) v.push_back(x);
+    return v;
+}
+
+// Spearman rank correlation (average ranks for ties)
+double spearman(const std::vector<double>& a, const std::vector<double>& b) {
+    const size_t n = a.size();
+    if (n < 3) return 0;
+    auto ranks = [&](const std::vector<double>& v) {
+        std::vector<size_t> idx(n);
+        std::iota(idx.begin(), idx.end(), 0);
+        std::stable_sort(idx.begin(), idx.end(), [&](size_t i, size_t j) { return v[i] < v[j]; });
+        std::vector<double> r(n);
+        for (size_t i = 0; i < n;) {
+            size_t j = i;
+            while (j + 1 < n && v[idx[j + 1]] == v[idx[i]]) ++j;
+            for (size_t k = i; k <= j; ++k) r[idx[k]] = 0.5 * (i + j);
+            i = j + 1;
+        }
+        return r;
+    };
+    const auto ra = ranks(a), rb = ranks(b);
+    const double m = 0.5 * (n - 1);
+    double sab = 0, saa = 0, sbb = 0;
+    for (size_t i = 0; i < n; ++i) {
+        sab += (ra[i] - m) * (rb[i] - m);
+        saa += (ra[i] - m) * (ra[i] - m);
+        sbb += (rb[i] - m) * (rb[i] - m);
+    }
+    return saa > 0 && sbb > 0 ? sab / std::sqrt(saa * sbb) : 0;
+}
+
+// Pearson correlation of two equally long vectors
+double pearson(const double* a, const double* b, size_t n) {
+    double ma = 0, mb = 0;
+    for (size_t i = 0; i < n; ++i) { ma += a[i]; mb += b[i]; }
+    ma /= n; mb /= n;
+    double sab = 0, saa = 0, sbb = 0;
+    for (size_t i = 0; i < n; ++i) {
+        sab += (a[i] - ma) * (b[i] - mb);
+        saa += (a[i] - ma) * (a[i] - ma);
+        sbb += (b[i] - mb) * (b[i] - mb);
+    }
+    return sab / std::sqrt(saa * sbb);
+}
+
+// printf to the log file and to stdout
+__attribute__((format(printf, 2, 3))) void both(FILE* log, const char* fmt, ...) {
+    va_list a, b;
+    va_start(a, fmt);
+    va_copy(b, a);
+    std::vfprintf(log, fmt, a);
+    std::vprintf(fmt, b);
+    va_end(b);
+    va_end(a);
+}
+
+// Feature / target standardisation for the MLPs
+struct Scaler {
+    std::vector<double> mu, sd;
+    void fit(const std::vector<float>& X, int n, int d) {
+        mu.assign(d, 0);
+        sd.assign(d, 0);
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < d; ++k) mu[k] += X[(size_t)i * d + k];
+        for (int k = 0; k < d; ++k) mu[k] /= n;
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < d; ++k) {
+                const double e = X[(size_t)i * d + k] - mu[k];
+                sd[k] += e * e;
+            }
+        for (int k = 0; k < d; ++k) sd[k] = std::sqrt(sd[k] / n) + 1e-8;
+    }
+    void apply(std::vector<float>& X, int d) const {
+        for (size_t i = 0; i < X.size(); ++i) X[i] = (float)((X[i] - mu[i % d]) / sd[i % d]);
+    }
+};
+
+}  // namespace
+
+int main(int argc, char* argv[]) {
+    if (argc != 20) {
+        std::printf("Usage: ./run_cmaes <sc_path> <out_prefix|'same'> <fc_tril_path> "
+                    "<fcd_tril_path> <maps_path> <n_regions> <G_min[-G_max]> <wee_min[-wee_max]> "
+                    "<wei_min[-wei_max]> <wie_min[-wie_max]> <het_params> <time_steps> <BOLD_TR> "
+                    "<fcd_step> <fcd_window> <rand_seed> <lambda> <itMax> <SeedMW>\n");
+        return 0;
+    }
+    // ---- arguments (run_CMAES.cpp:192-219) ----
+    std::string sc_path = argv[1], out_path = argv[2], fc_tril_path = argv[3], fcd_tril_path = argv[4],
+                maps_path = argv[5];
+    const int nodes = std::atoi(argv[6]);
+    std::map<std::string, std::string> param_strings = {
+        {"G", argv[7]}, {"wee", argv[8]}, {"wei", argv[9]}, {"wie", argv[10]}};
+    const std::string het_params_str = argv[11];
+    const int time_steps = std::atoi(argv[12]), BOLD_TR = std::atoi(argv[13]),
+              window_step = std::atoi(argv[14]), window_size = std::atoi(argv[15]),
+              rand_seed = std::atoi(argv[16]), lambda = std::atoi(argv[17]), itMax = std::atoi(argv[18]),
+              SeedMW = std::atoi(argv[19]);
+    if (nodes < 2 || lambda < 2) die("n_regions and lambda must be >= 2");
+
+    // ---- free / fixed parameters (run_CMAES.cpp:226-253): a '-' after the
+    // first character marks a range min-max ----
+    std::map<std::string, double> params, pmin, pmax, pdiff;
+    std::vector<std::string> free_params;
+    for (auto& kv : param_strings) {
+        const std::string rest = kv.second.substr(1);
+        const size_t dash = rest.find('-');
+        if (dash != std::string::npos) {
+            params[kv.first] = -100;
+            pmin[kv.first] = std::atof(kv.second.substr(0, dash + 1).c_str());
+            pmax[kv.first] = std::atof(kv.second.substr(dash + 2).c_str());
+            pdiff[kv.first] = pmax[kv.first] - pmin[kv.first];
+            free_params.push_back(kv.first);
+        } else {
+            params[kv.first] = std::atof(kv.second.c_str());
+        }
+    }
+    // heterogeneous local parameters (run_CMAES.cpp:255-266)
+    const std::vector<std::string> local_params = {"wee", "wie", "wei"};
+    std::vector<std::string> het_params, homo_params;
+    for (auto& p : local_params)
+        (het_params_str.find(p) != std::string::npos ? het_params : homo_params).push_back(p);
+    const bool heterogeneous = !het_params.empty();
+    // maps and their scale ranges (run_CMAES.cpp:268-310)
+    std::vector<std::vector<double>> maps;
+    if (heterogeneous) {
+        std::ifstream mf(maps_path);
+        if (!mf) die("Error opening maps file");
+        const double scale_max_minmax = env_d("BNM_SCALE_MAX_MINMAX", 1.0);
+        std::string line;
+        while (std::getline(mf, line)) {
+            std::istringstream iss(line);
+            std::vector<double> m;
+            double v;
+            while (iss >> v) m.push_back(v);
+            if (m.empty()) continue;
+            if ((int)m.size() != nodes) die("maps file: each map needs n_regions values");
+            const double mx = *std::max_element(m.begin(), m.end()), mn = *std::min_element(m.begin(), m.end());
+            double smin, smax;
+            if (mn == 0 && mx == 1) {
+                smin = 0;
+                smax = scale_max_minmax;
+            } else {
+                smin = std::ceil(-1 / mx / 0.01) * 0.01;
+                smax = std::floor(-1 / mn / 0.01) * 0.01;
+            }
+            const int mi = (int)maps.size();
+            maps.push_back(m);
+            for (auto& hp : het_params) {
+                const std::string name = hp + "scale_" + std::to_string(mi);
+                params[name] = -100;
+                pmin[name] = smin;
+                pmax[name] = smax;
+                pdiff[name] = smax - smin;
+                free_params.push_back(name);
+            }
+        }
+    }
+    const int D = (int)free_params.size();
+    if (D < 1) die("no free parameter");
+    // empirical FC / FCD (run_CMAES.cpp:312-345)
+    std::vector<double> emp_fc = read_values(fc_tril_path, "empirical FC");
+    const bool no_fcd = fcd_tril_path == "no_fcd";
+    std::vector<double> emp_fcd;
+    if (!no_fcd) emp_fcd = read_values(fcd_tril_path, "empirical FCD");
+    const bool do_fic = params["wie"] == 0;
+    if (do_fic && std::find(het_params.begin(), het_params.end(), "wie") != het_params.end())
+        die("Error: wIE cannot be a heterogeneous parameter in FIC+ model");
+    // SC (run_CMAES.cpp:349-372)
+    std::vector<double> SC((size_t)nodes * nodes);
+    {
+        FILE* fp = std::fopen(sc_path.c_str(), "r");
+        if (!fp) die("Error opening SC file");
+        for (size_t i = 0; i < SC.size(); ++i)
+            if (std::fscanf(fp, "%lf", &SC[i]) != 1) die("SC file: too few values");
+        std::fclose(fp);
+    }
+    std::vector<double> strength(nodes, 0.0);
+    for (int j = 0; j < nodes; ++j)
+        for (int k = 0; k < nodes; ++k) strength[j] += SC[(size_t)j * nodes + k];
+
+    // ---- output folders and names (run_CMAES.cpp:394-424) ----
+    if (out_path == "same") out_path = sc_path.substr(0, sc_path.size() >= 4 ? sc_path.size() - 4 : 0);
+    mkdir(out_path.c_str(), 0751);
+    std::string sims_dir = out_path + "/sims", cmaes_dir = out_path + "/cmaes";
+    if (heterogeneous) { sims_dir += "_multimaps"; cmaes_dir += "_multimaps"; }
+    sims_dir += "_gpu";
+    cmaes_dir += "_gpu";
+    mkdir(sims_dir.c_str(), 0751);
+    mkdir(cmaes_dir.c_str(), 0751);
+    const size_t pos = fc_tril_path.find_last_of('/');
+    const size_t base_start = pos == std::string::npos ? 0 : pos + 1;
+    // basename without its "_desc-FCtril.txt" suffix (16 characters); a name
+    // shorter than that is kept whole
+    std::string out_prefix = fc_tril_path.substr(base_start);
+    if (fc_tril_path.size() >= base_start + 16) out_prefix = fc_tril_path.substr(base_start, fc_tril_path.size() - base_start - 16);
+    for (auto& kv : param_strings) out_prefix += "_" + kv.first + "_" + kv.second;
+    out_prefix += "_het-" + het_params_str + "_SeedMW-" + std::to_string(SeedMW) + "_SeedSim-" +
+                  std::to_string(rand_seed) + "_n-" + std::to_string(itMax) + "x" + std::to_string(lambda);
+    const std::string log_path = cmaes_dir + "/" + out_prefix + ".txt";
+    const std::string sims_prefix = sims_dir + "/" + out_prefix + "_";
+    FILE* log = std::fopen(log_path.c_str(), "w");
+    if (!log) die("cannot open the log file " + log_path);
+    const double sigma0 = env_d("BNM_CMAES_SIGMA", 0.5);
+    const int variante = env_i("BNM_CMAES_VARIANTE", 6);
+    both(log, "Running %dD CMAES\nSC: %s\nFC: %s\n", D, sc_path.c_str(), fc_tril_path.c_str());
+    if (heterogeneous) both(log, "Maps: %s\n", maps_path.c_str());
+    both(log, "Free parameters: ");
+    for (auto& kv : pmin) both(log, "%s (%f-%f) ", kv.first.c_str(), pmin[kv.first], pmax[kv.first]);
+    both(log, "\nFCD step %d, FCD window %d, Simulation random seed %d, CMAES random seed %d, sigma %f, "
+         "alphacov %f, gamma_scale %f, bound_soft_edge %f, Variante %d, early stop gens %d, early stop tol %f\n",
+         window_step, window_size, rand_seed, SeedMW, sigma0, 2.0, 2.0, 0.0, variante, 30, 1e-3);
+
+    // ---- simulation context ----
+    bnm_config cfg;
+    check(bnm_default_config(&cfg), "bnm_default_config");
+    cfg.nodes = nodes;
+    cfg.n_sims = lambda;
+    cfg.time_steps = time_steps;
+    cfg.bold_tr = BOLD_TR;
+    cfg.window_size = window_size;
+    cfg.window_step = window_step;
+    cfg.do_fic = do_fic;
+    cfg.adjust_fic = do_fic && cfg.adjust_fic;
+    cfg.calculate_fic_penalty = do_fic;
+    cfg.no_fcd = no_fcd;
+    cfg.noise_period_ms = (long long)time_steps + 1;
+    const std::string prec = env_s("BNM_PRECISION", "f64");
+    cfg.precision = prec == "f32" ? BNM_PREC_F32 : prec == "bf16x3" ? BNM_PREC_BF16X3 : BNM_PREC_F64;
+    cfg.fc_precision = cfg.precision == BNM_PREC_F64 ? BNM_FC_F64 : BNM_FC_F32;
+    check(bnm_finalize_config(&cfg), "bnm_finalize_config");
+    const int max_trials = cfg.max_fic_trials;
+
+    // noise: (time_steps + 1) * 10 * N * 2 draws (run_CMAES.cpp:640-657)
+    std::printf("Noise precalculation...\n");
+    auto t0 = Clock::now();
+    const long long noise_len = ((long long)time_steps + 1) * 10 * nodes * 2;
+    std::vector<float> noise(noise_len);
+    if (env_i("BNM_NOISE_CHUNKED", 0)) check(bnm_make_noise_chunked(rand_seed, noise_len, noise.data(), 0), "noise");
+    else check(bnm_make_noise(rand_seed, noise_len, noise.data()), "noise");
+    std::printf("took %lf s\n", std::chrono::duration<double>(Clock::now() - t0).count());
+
+    int st = 0;
+    bnm_ctx* ctx = bnm_ctx_create(&cfg, &st);
+    if (!ctx) die(std::string("bnm_ctx_create: ") + bnm_last_error());
+    check(bnm_ctx_set_sc(ctx, SC.data()), "set_sc");
+    check(bnm_ctx_set_noise(ctx, noise.data(), noise_len), "set_noise");
+    check(bnm_ctx_set_empirical(ctx, emp_fc.data(), (int)emp_fc.size(), no_fcd ? nullptr : emp_fcd.data(),
+                                (int)emp_fcd.size()),
+          "set_empirical");
+    const int n_pairs = bnm_n_pairs(nodes, cfg.exc_interhemispheric);
+    if ((int)emp_fc.size() != n_pairs) die("empirical FC size does not match n_regions");
+    double emp_fc_mean = 0;
+    for (double v : emp_fc) emp_fc_mean += v;
+    emp_fc_mean /= emp_fc.size();
+
+    // ---- parameter lists of one candidate (calculate_params, run_CMAES.cpp:61-143) ----
+    std::vector<double> G_list(lambda), wEE((size_t)lambda * nodes), wEI((size_t)lambda * nodes),
+        wIE((size_t)lambda * nodes);
+    auto fill_params = [&](const double* x, int b, double* G_o, double* ee_o, double* ei_o, double* ie_o) {
+        std::map<std::string, double> cur = params;
+        for (int f = 0; f < D; ++f) cur[free_params[f]] = pdiff[free_params[f]] * x[f] + pmin[free_params[f]];
+        G_o[b] = cur["G"];
+        std::vector<double> scales(maps.size()), flat;
+        for (auto& m : maps) flat.insert(flat.end(), m.begin(), m.end());
+        for (auto& hp : het_params) {
+            for (size_t m = 0; m < maps.size(); ++m) scales[m] = cur[hp + "scale_" + std::to_string(m)];
+            double* o = hp == "wee" ? ee_o : hp == "wei" ? ei_o : ie_o;
+            check(bnm_regional_params(nodes, (int)maps.size(), flat.data(), cur[hp], scales.data(),
+                                      o + (size_t)b * nodes),
+                  "regional params");
+        }
+        for (auto& hp : homo_params) {
+            double* o = hp == "wee" ? ee_o : hp == "wei" ? ei_o : ie_o;
+            for (int j = 0; j < nodes; ++j) o[(size_t)b * nodes + j] = cur[hp];
+        }
+        return cur;
+    };
+
+    // ---- surrogate set-up ----
+    const bool surrogate = env_i("BNM_SURROGATE", 0) != 0;
+    const int pool_factor = std::max(1, env_i("BNM_SURR_POOL", 4));
+    const int warmup = std::max(1, env_i("BNM_SURR_WARMUP", 1));
+    const double min_rho = env_d("BNM_SURR_MIN_RHO", 0.3);
+    const int mlp_hidden = env_i("BNM_SURR_HIDDEN", 256), mlp_depth = env_i("BNM_SURR_DEPTH", 4);
+    bnm_mlp* fic_net = nullptr;
+    Scaler fic_xs;
+    double fic_ys = 1.0;
+    double fic_train_sim_s = 0;
+    auto an_fic = [&](int B, const double* G_, const double* ee, const double* ei, double* out) {
+        std::vector<int> un(B);
+        check(bnm_analytical_fic(nodes, B, SC.data(), G_, ee, ei, out, un.data()), "analytical FIC");
+    };
+    auto fic_features = [&](int B, const double* G_, const double* ee, const double* ei, const double* an) {
+        std::vector<float> X((size_t)B * nodes * 5);
+        for (int b = 0; b < B; ++b)
+            for (int j = 0; j < nodes; ++j) {
+                float* r = &X[((size_t)b * nodes + j) * 5];
+                const size_t q = (size_t)b * nodes + j;
+                r[0] = (float)G_[b];
+                r[1] = (float)ee[q];
+                r[2] = (float)ei[q];
+                r[3] = (float)strength[j];
+                r[4] = (float)an[q];
+            }
+        return X;
+    };
+    if (surrogate && do_fic && cfg.adjust_fic) {
+        // FIC-phase training set: parameter sets drawn uniformly in the search box
+        const int n_train = env_i("BNM_SURR_FIC_SIMS", 8 * lambda);
+        const int t_max = env_i("BNM_SURR_FIC_TRIALS", 50);
+        bnm_config fc = cfg;
+        fc.n_sims = n_train;
+        fc.time_steps = cfg.i_sampling_end;
+        fc.sim_only = 1;
+        fc.n_vols_remove = 0;
+        fc.noise_period_ms = fc.time_steps + 1;
+        fc.max_fic_trials = t_max;
+        fc.calculate_fic_penalty = 0;
+        bnm_ctx* fctx = bnm_ctx_create(&fc, &st);
+        if (!fctx) die(std::string("FIC training context: ") + bnm_last_error());
+        check(bnm_ctx_set_sc(fctx, SC.data()), "set_sc");
+        check(bnm_ctx_set_noise(fctx, noise.data(), noise_len), "set_noise");
+        std::vector<double> g(n_train), ee((size_t)n_train * nodes), ei((size_t)n_train * nodes),
+            ie((size_t)n_train * nodes, 0.0), an((size_t)n_train * nodes), wf((size_t)n_train * nodes);
+        std::mt19937_64 rng(1000003ull + SeedMW);
+        std::uniform_real_distribution<double> u(0, 1);
+        std::vector<double> x(D);
+        for (int b = 0; b < n_train; ++b) {
+            for (int f = 0; f < D; ++f) x[f] = u(rng);
+            fill_params(x.data(), b, g.data(), ee.data(), ei.data(), ie.data());
+        }
+        check(bnm_ctx_set_params(fctx, g.data(), ee.data(), ei.data(), ie.data(), nullptr), "set_params");
+        t0 = Clock::now();
+        check(bnm_ctx_run(fctx), "FIC training run");
+        std::vector<int> failed(n_train), trials(n_train);
+        check(bnm_ctx_get_outputs(fctx, nullptr, nullptr, wf.data(), nullptr, nullptr, nullptr, nullptr, nullptr,
+                                  nullptr, nullptr, failed.data(), trials.data(), nullptr, nullptr),
+              "outputs");
+        bnm_ctx_destroy(fctx);
+        an_fic(n_train, g.data(), ee.data(), ei.data(), an.data());
+        std::vector<float> Xall = fic_features(n_train, g.data(), ee.data(), ei.data(), an.data()), X, Y;
+        int rows = 0;
+        double tr_sum = 0;
+        for (int b = 0; b < n_train; ++b) {
+            tr_sum += trials[b];
+            fic_train_sim_s += (fc.time_steps + 1 + (double)trials[b] * (fc.time_steps + 1)) / 1000.0;
+            if (failed[b]) continue;
+            for (int j = 0; j < nodes; ++j) {
+                const size_t q = (size_t)b * nodes + j;
+                X.insert(X.end(), &Xall[q * 5], &Xall[q * 5] + 5);
+                Y.push_back((float)(wf[q] - an[q]));
+                ++rows;
+            }
+        }
+        if (rows < nodes) die("FIC training set: every simulation failed");
+        fic_xs.fit(X, rows, 5);
+        fic_xs.apply(X, 5);
+        double ysd = 0;
+        for (float v : Y) ysd += (double)v * v;
+        fic_ys = std::sqrt(ysd / rows) + 1e-6;
+        for (float& v : Y) v = (float)(v / fic_ys);
+        fic_net = bnm_mlp_create(5, mlp_hidden, mlp_depth, 1, 1, 7, &st);
+        if (!fic_net) die(std::string("MLP: ") + bnm_mlp_last_error());
+        check(bnm_mlp_set_data(fic_net, X.data(), Y.data(), rows), "mlp data");
+        const int epochs = env_i("BNM_SURR_FIC_EPOCHS", 60);
+        std::vector<float> losses(epochs);
+        check(bnm_mlp_train(fic_net, epochs, 1024, 1e-3f, 11, losses.data()), "mlp train");
+        both(log, "Surrogate FIC initialiser: %d sets (%d converged), mean trials %.3f, %.1f simulated s; "
+             "%d rows, loss %.4g -> %.4g, took %lf s\n",
+             n_train, rows / nodes, tr_sum / n_train, fic_train_sim_s, rows, losses.front(), losses.back(),
+             std::chrono::duration<double>(Clock::now() - t0).count());
+    }
+    bnm_mlp* fit_net = nullptr;
+    std::vector<float> fit_X, fit_Y;   // unit-box coordinates -> (r, KS, |dmean|, fic penalty)
+    if (surrogate) {
+        fit_net = bnm_mlp_create(D, mlp_hidden, mlp_depth, 4, 1, 13, &st);
+        if (!fit_net) die(std::string("MLP: ") + bnm_mlp_last_error());
+    }
+
+    bnm_ctx* sctx = nullptr;
+    if (fic_net) {
+        bnm_config sc = cfg;
+        sc.analytical_fic = 0;
+        sctx = bnm_ctx_create(&sc, &st);
+        if (!sctx) die(std::string("context: ") + bnm_last_error());
+        check(bnm_ctx_set_sc(sctx, SC.data()), "set_sc");
+        check(bnm_ctx_set_noise(sctx, noise.data(), noise_len), "set_noise");
+        check(bnm_ctx_set_empirical(sctx, emp_fc.data(), (int)emp_fc.size(), no_fcd ? nullptr : emp_fcd.data(),
+                                    (int)emp_fcd.size()),
+              "set_empirical");
+    }
+
+    // ---- the search (run_CMAES.cpp:714-832) ----
+    bnm_cmaes* es = bnm_cmaes_create(D, lambda, (unsigned)SeedMW, sigma0, variante);
+    if (!es) die("CMA-ES set-up failed (variant must be 6 or 16)");
+    std::vector<double> x((size_t)lambda * D), corr(lambda), pen(lambda), raw(lambda), penalized(lambda),
+        fc((size_t)lambda * n_pairs), wfic((size_t)lambda * nodes);
+    std::vector<int> failed(lambda), trials(lambda);
+    FILE* prog = nullptr;
+    if (env_i("BNM_PROGRESS_CSV", 1)) {
+        prog = std::fopen((cmaes_dir + "/" + out_prefix + "_progress.csv").c_str(), "w");
+        if (prog)
+            std::fprintf(prog, "iteration,full_sims,simulated_s,gen_best_cost,best_cost,mean_fic_trials,"
+                               "max_fic_trials,surrogate_used,surrogate_rho,gpu_ms\n");
+    }
+    long long full_sims = 0;
+    double sim_seconds = 0, last_rho = -1, best_so_far = INFINITY;
+    bool used_surr = false;
+    while (bnm_cmaes_iteration(es) < itMax) {
+        const int it = bnm_cmaes_iteration(es);
+        both(log, "Iteration %d\n", it);
+        std::vector<double> pred_cost;
+        used_surr = false;
+        if (surrogate && it > warmup && last_rho >= min_rho && pool_factor > 1) {
+            // pre-screen a pool of pool_factor x lambda candidates
+            const int n_pool = pool_factor * lambda;
+            std::vector<double> pool((size_t)n_pool * D);
+            check(bnm_cmaes_ask_pool(es, n_pool, 7919u * it + (unsigned)SeedMW, pool.data()), "ask_pool");
+            std::vector<float> px(pool.begin(), pool.end()), py((size_t)n_pool * 4);
+            check(bnm_mlp_predict(fit_net, px.data(), n_pool, py.data()), "surrogate predict");
+            std::vector<double> c(n_pool);
+            for (int i = 0; i < n_pool; ++i) {
+                const float* q = &py[(size_t)i * 4];
+                c[i] = 1 - q[0] + q[1] + q[2] + (do_fic ? q[3] : 0.0);
+            }
+            std::vector<int> idx(n_pool);
+            std::iota(idx.begin(), idx.end(), 0);
+            std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return c[a] < c[b]; });
+            idx.resize(lambda);
+            check(bnm_cmaes_select(es, idx.data()), "select");
+            for (int i = 0; i < lambda; ++i) pred_cost.push_back(c[idx[i]]);
+            used_surr = true;
+        }
+        check(bnm_cmaes_ask(es, x.data()), "ask");
+        for (int b = 0; b < lambda; ++b)
+            fill_params(&x[(size_t)b * D], b, G_list.data(), wEE.data(), wEI.data(), wIE.data());
+        const double* wfic_in = nullptr;
+        std::vector<double> an, init;
+        if (fic_net) {
+            an.resize((size_t)lambda * nodes);
+            an_fic(lambda, G_list.data(), wEE.data(), wEI.data(), an.data());
+            std::vector<float> X = fic_features(lambda, G_list.data(), wEE.data(), wEI.data(), an.data());
+            fic_xs.apply(X, 5);
+            std::vector<float> y((size_t)lambda * nodes);
+            check(bnm_mlp_predict(fic_net, X.data(), lambda * nodes, y.data()), "FIC predict");
+            init.resize(an.size());
+            for (size_t q = 0; q < an.size(); ++q) init[q] = an[q] + fic_ys * y[q];
+            wfic_in = init.data();
+        }
+        // with the learned initialiser the context takes w_IE_fic as given
+        bnm_ctx* rctx = fic_net ? sctx : ctx;
+        check(bnm_ctx_set_params(rctx, G_list.data(), wEE.data(), wEI.data(), wIE.data(), wfic_in), "params");
+        t0 = Clock::now();
+        check(bnm_ctx_run(rctx), "run");
+        check(bnm_ctx_get_outputs(rctx, corr.data(), pen.data(), wfic.data(), nullptr, nullptr, nullptr, nullptr,
+                                  nullptr, nullptr, nullptr, failed.data(), trials.data(), fc.data(), nullptr),
+              "outputs");
+        const double took = std::chrono::duration<double>(Clock::now() - t0).count();
+        std::printf("took %lf s\n", took);
+        full_sims += lambda;
+        double tr_sum = 0;
+        int tr_max = 0;
+        for (int b = 0; b < lambda; ++b) {
+            tr_sum += trials[b];
+            tr_max = std::max(tr_max, trials[b]);
+            sim_seconds += (time_steps + 1 + (do_fic ? (double)trials[b] * (cfg.i_sampling_end + 1) : 0.0)) / 1000.0;
+            raw[b] = corr[b] + (do_fic ? pen[b] : 0.0);   // run_CMAES.cpp:787-793
+        }
+        // surrogate bookkeeping: targets of every simulated candidate
+        if (fit_net) {
+            std::vector<double> actual(lambda);
+            for (int b = 0; b < lambda; ++b) {
+                const double* f = &fc[(size_t)b * n_pairs];
+                double m = 0;
+                for (int i = 0; i < n_pairs; ++i) m += f[i];
+                m /= n_pairs;
+                const double r = pearson(emp_fc.data(), f, n_pairs);
+                const double dm = cfg.use_fc_diff ? std::fabs(emp_fc_mean - m) : 0.0;
+                // a simulation whose FC/FCD had a NaN costs max_cost: targets
+                // r = 0 and KS = |dmean| = (max_cost - 1) / 2 add up to it
+                const bool ok = std::isfinite(r) && corr[b] < cfg.max_cost;
+                const double half = 0.5 * (cfg.max_cost - 1);
+                const double ks = ok ? corr[b] - (1 - r) - dm : half;
+                for (int f2 = 0; f2 < D; ++f2) fit_X.push_back((float)x[(size_t)b * D + f2]);
+                fit_Y.push_back((float)(ok ? r : 0.0));
+                fit_Y.push_back((float)ks);
+                fit_Y.push_back((float)(ok ? dm : half));
+                fit_Y.push_back((float)(do_fic ? pen[b] : 0.0));
+                actual[b] = raw[b];
+            }
+            if (used_surr) last_rho = spearman(pred_cost, actual);
+            const int n = (int)(fit_Y.size() / 4);
+            check(bnm_mlp_set_data(fit_net, fit_X.data(), fit_Y.data(), n), "surrogate data");
+            const int steps = env_i("BNM_SURR_FIT_STEPS", 1500);
+            const int batch = std::min(n, 256);
+            const int epochs = std::max(1, steps / std::max(1, n / batch));
+            check(bnm_mlp_train(fit_net, epochs, batch, 1e-3f, 17u + it, nullptr), "surrogate train");
+            if (!used_surr) last_rho = 1.0;   // the first fit enables pre-screening
+        }
+        const int stopped = bnm_cmaes_tell(es, raw.data(), penalized.data());
+        if (stopped < 0) die("CMA-ES tell failed");
+        double gen_best = INFINITY;
+        for (int i = 0; i < lambda; ++i) {
+            gen_best = std::min(gen_best, raw[i]);
+            both(log, "\t IndPar %d: ", i);
+            std::map<std::string, double> cur = params;
+            for (int f = 0; f < D; ++f) {
+                cur[free_params[f]] = pdiff[free_params[f]] * x[(size_t)i * D + f] + pmin[free_params[f]];
+                both(log, "%s=%f,", free_params[f].c_str(), cur[free_params[f]]);
+            }
+            if (do_fic) both(log, "gof=%f,fic_penalty=%f,penalized_cost=%f\n", -corr[i], pen[i], penalized[i]);
+            else both(log, "gof=%f,penalized_cost=%f\n", -corr[i], penalized[i]);
+        }
+        best_so_far = std::min(best_so_far, gen_best);
+        if (prog)
+            std::fprintf(prog, "%d,%lld,%.3f,%.9g,%.9g,%.4f,%d,%d,%.4f,%.3f\n", it, full_sims, sim_seconds, gen_best,
+                         best_so_far, tr_sum / lambda, tr_max, used_surr ? 1 : 0, used_surr ? last_rho : NAN,
+                         took * 1e3);
+        if (surrogate && used_surr) both(log, "surrogate: rank correlation of predicted and simulated cost %.3f\n", last_rho);
+        if (stopped) break;   // early stop, or the covariance lost definiteness
+    }
+    if (prog) std::fclose(prog);
+
+    // ---- best feasible point, re-run with the plain method (run_CMAES.cpp:835-906) ----
+    std::vector<double> bx(D);
+    double best_raw = 0;
+    check(bnm_cmaes_best(es, bx.data(), &best_raw), "best");
+    both(log, "Best feasible parameters: ");
+    {
+        std::map<std::string, double> cur = params;
+        for (int f = 0; f < D; ++f) {
+            cur[free_params[f]] = pdiff[free_params[f]] * bx[f] + pmin[free_params[f]];
+            both(log, "%s=%f,", free_params[f].c_str(), cur[free_params[f]]);
+        }
+    }
+    bnm_config one = cfg;
+    one.n_sims = 1;
+    bnm_ctx* bctx = bnm_ctx_create(&one, &st);
+    if (!bctx) die(std::string("context: ") + bnm_last_error());
+    check(bnm_ctx_set_sc(bctx, SC.data()), "set_sc");
+    check(bnm_ctx_set_noise(bctx, noise.data(), noise_len), "set_noise");
+    check(bnm_ctx_set_empirical(bctx, emp_fc.data(), (int)emp_fc.size(), no_fcd ? nullptr : emp_fcd.data(),
+                                (int)emp_fcd.size()),
+          "set_empirical");
+    std::vector<double> g1(1), ee1(nodes), ei1(nodes), ie1(nodes);
+    fill_params(bx.data(), 0, g1.data(), ee1.data(), ei1.data(), ie1.data());
+    check(bnm_ctx_set_params(bctx, g1.data(), ee1.data(), ei1.data(), ie1.data(), nullptr), "params");
+    check(bnm_ctx_run(bctx), "best run");
+    const int ots = bnm_output_ts(time_steps, BOLD_TR);
+    std::vector<double> c1(1), p1(1), w1(nodes), bold((size_t)ots * nodes), ext(6 * (size_t)nodes);
+    int f1 = 0, t1 = 0;
+    check(bnm_ctx_get_outputs(bctx, c1.data(), p1.data(), w1.data(), bold.data(), &ext[0], &ext[nodes],
+                              &ext[2 * nodes], &ext[3 * nodes], &ext[4 * nodes], &ext[5 * nodes], &f1, &t1, nullptr,
+                              nullptr),
+          "best outputs");
+    bnm_ctx_destroy(bctx);
+    if (do_fic) both(log, "gof=%f,fic_penalty=%f,gof_fic_penalty=%f\n", -c1[0], p1[0], -best_raw);
+    else both(log, "gof=%f\n", -best_raw);
+    // best-simulation files (bnm.cu:771-833)
+    auto wfile = [&](const std::string& name, const double* v, int n) {
+        FILE* f = std::fopen((sims_prefix + name).c_str(), "w");
+        if (!f) die("cannot write " + sims_prefix + name);
+        for (int j = 0; j < n; ++j) std::fprintf(f, "%f ", v[j]);
+        std::fclose(f);
+    };
+    {
+        FILE* f = std::fopen((sims_prefix + "bold.txt").c_str(), "w");
+        if (!f) die("cannot write " + sims_prefix + "bold.txt");
+        for (int i = 0; i < ots; ++i) {
+            for (int j = 0; j < nodes; ++j) std::fprintf(f, "%f ", bold[(size_t)i * nodes + j]);
+            std::fprintf(f, "\n");
+        }
+        std::fclose(f);
+    }
+    const char* ext_names[6] = {"S_E.txt", "I_E.txt", "r_E.txt", "S_I.txt", "I_I.txt", "r_I.txt"};
+    for (int k = 0; k < 6; ++k) wfile(ext_names[k], &ext[(size_t)k * nodes], nodes);
+    wfile("w_EE.txt", ee1.data(), nodes);
+    wfile("w_EI.txt", ei1.data(), nodes);
+    if (do_fic) {
+        wfile("w_IE.txt", w1.data(), nodes);
+        if (cfg.adjust_fic) {
+            FILE* f = std::fopen((sims_prefix + "fic_failed.txt").c_str(), "w");
+            if (f) { std::fprintf(f, "%d", f1); std::fclose(f); }
+            f = std::fopen((sims_prefix + "fic_ntrials.txt").c_str(), "w");
+            if (f) { std::fprintf(f, "%d of max %d", t1, max_trials); std::fclose(f); }
+        }
+    }
+    if (surrogate)
+        both(log, "Surrogate summary: full simulations %lld, simulated s %.1f (+%.1f for the FIC training set)\n",
+             full_sims, sim_seconds, fic_train_sim_s);
+    std::fclose(log);
+    bnm_cmaes_destroy(es);
+    if (fic_net) bnm_mlp_destroy(fic_net);
+    if (fit_net) bnm_mlp_destroy(fit_net);
+    if (sctx) bnm_ctx_destroy(sctx);
+    bnm_ctx_destroy(ctx);
+    return 0;
+}

@@ -123,6 +123,15 @@ int bnm_window_bounds(const bnm_config* cfg, int* starts, int* ends);
 int bnm_regional_params(int nodes, int n_maps, const double* maps, double base,
                         const double* scales, double* out);
 
+/* Analytical FIC wIE (fic.cpp:130-207 of the reference, called per particle by
+ * run_CMAES.cpp:144-174) for n_sims parameter sets, on the device: per node the
+ * Brent root of the inhibitory steady state on [0, 2] (relative tolerance
+ * 1e-3), then wIE_j; an unstable set (J < 0 or no root) gets wIE = 1 at every
+ * node and unstable[b] = 1.  SC [N*N], G [B], wEE/wEI/wIE_out [B*N],
+ * unstable [B] (optional). */
+int bnm_analytical_fic(int nodes, int n_sims, const double* SC, const double* G, const double* wEE,
+                       const double* wEI, double* wIE_out, int* unstable);
+
 /* Host noise array as the reference's driver makes it (run_CMAES.cpp:640-657):
  * `count` draws of std::normal_distribution<float>(0,1) over std::mt19937(seed). */
 int bnm_make_noise(unsigned seed, long long count, float* out);

@@ -91,6 +91,20 @@ def test_analytical_fic_matches_oracle(oracle_mod, case):
         np.testing.assert_allclose(out["wIE_fic"][b], w, rtol=1e-12, atol=1e-13)
 
 
+def test_analytical_fic_entry_point_matches_oracle(oracle_mod, case):
+    """bnm_analytical_fic (stand-alone entry point) on the same sets, plus an
+    unstable set (negative wEE) that must come back as wIE = 1."""
+    G = np.append(case["G"], 1.0)
+    wEE = np.vstack([case["wEE"], np.full(N, -5.0)])
+    wEI = np.vstack([case["wEI"], np.full(N, 0.15)])
+    w, un = bnm.analytical_fic(case["SC"], G, wEE, wEI)
+    for b in range(G.size):
+        wo, uo = oracle_mod.analytical_fic(case["SC"], G[b], wEE[b], wEI[b])
+        assert bool(un[b]) == bool(uo)
+        np.testing.assert_allclose(w[b], np.ones(N) if uo else wo, rtol=1e-12, atol=1e-13)
+    assert un[-1]
+
+
 SPLITS = [(1, 1), (2, 1), (4, 1), (1, 2), (2, 2)]
 
 
