@@ -69,6 +69,7 @@ struct bnm_ctx {
     long long nwp = 0;
     bool f64 = true;      // integration precision
     bool fc64 = true;     // Gram precision
+    bool fcbf16 = false;  // BF16 Gram operands (FP32 accumulation and storage)
     hipStream_t st = nullptr;
     double* SC64 = nullptr;
     float* SC32 = nullptr;
@@ -396,7 +397,13 @@ bnm_ctx* bnm_ctx_create(const bnm_config* cfg, int* status) {
     x->B = c.n_sims;
     x->ots = c.time_steps / c.bold_tr + 1;
     x->f64 = c.precision == BNM_PREC_F64;
+    if (c.fc_precision < BNM_FC_F64 || c.fc_precision > BNM_FC_BF16) {
+        set(fail(BNM_E_ARG, "unknown fc_precision"));
+        delete x;
+        return nullptr;
+    }
     x->fc64 = c.fc_precision == BNM_FC_F64;
+    x->fcbf16 = c.fc_precision == BNM_FC_BF16;
     if (!c.sim_only) {
         x->np = bnm_n_pairs(c.nodes, c.exc_interhemispheric);
         x->nw = c.no_fcd ? 0 : bnm_n_windows(&c);
@@ -695,6 +702,10 @@ static int enqueue_run(bnm_ctx* x, hipStream_t st, bool timed) {
                 HIPCHK((launch_gram_fc<double, double, double>(
                     (const double*)x->bold, x->B, x->N, x->ots, x->nseg, x->seg_start, x->seg_len,
                     x->mean, x->inv, x->pidx, x->np, (double*)x->fc, (double*)x->wfc, st)));
+            else if (x->fcbf16)
+                HIPCHK(launch_gram_fc_bf16<double>(
+                    (const double*)x->bold, x->B, x->N, x->ots, x->nseg, x->seg_start, x->seg_len,
+                    x->mean, x->inv, x->pidx, x->np, (float*)x->fc, (float*)x->wfc, st));
             else
                 HIPCHK((launch_gram_fc<double, float, float>(
                     (const double*)x->bold, x->B, x->N, x->ots, x->nseg, x->seg_start, x->seg_len,
@@ -706,6 +717,10 @@ static int enqueue_run(bnm_ctx* x, hipStream_t st, bool timed) {
                 HIPCHK((launch_gram_fc<float, double, double>(
                     (const float*)x->bold, x->B, x->N, x->ots, x->nseg, x->seg_start, x->seg_len,
                     x->mean, x->inv, x->pidx, x->np, (double*)x->fc, (double*)x->wfc, st)));
+            else if (x->fcbf16)
+                HIPCHK(launch_gram_fc_bf16<float>(
+                    (const float*)x->bold, x->B, x->N, x->ots, x->nseg, x->seg_start, x->seg_len,
+                    x->mean, x->inv, x->pidx, x->np, (float*)x->fc, (float*)x->wfc, st));
             else
                 HIPCHK((launch_gram_fc<float, float, float>(
                     (const float*)x->bold, x->B, x->N, x->ots, x->nseg, x->seg_start, x->seg_len,
@@ -746,8 +761,12 @@ static int enqueue_run(bnm_ctx* x, hipStream_t st, bool timed) {
         } else {
             HIPCHK(launch_row_moments<float>((const float*)x->wfc, (long long)x->B * x->nw, x->np,
                                              x->wmean, x->winv, st));
-            HIPCHK((launch_gram_fcd<float, float, float>((const float*)x->wfc, x->B, x->nw, x->np,
-                                                         x->wmean, x->winv, (float*)x->fcd, st)));
+            if (x->fcbf16)
+                HIPCHK(launch_gram_fcd_bf16((const float*)x->wfc, x->B, x->nw, x->np, x->wmean, x->winv,
+                                            (float*)x->fcd, st));
+            else
+                HIPCHK((launch_gram_fcd<float, float, float>((const float*)x->wfc, x->B, x->nw, x->np,
+                                                             x->wmean, x->winv, (float*)x->fcd, st)));
         }
     }
     if (timed) HIPCHK(hipEventRecord(x->ev[4], st));

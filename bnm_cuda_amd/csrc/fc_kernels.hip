@@ -192,7 +192,126 @@ __device__ void gram_tile(const GramItem& it, int ti, int tj, const int* __restr
             }
 }
 
+// BF16 operands, FP32 accumulation (v_mfma_f32_16x16x32_bf16): z-scored
+// values are formed in FP64, rounded to bfloat16 (nearest even) in the tile
+// loader and staged k-contiguous per row, so each lane fetches its 8
+// consecutive k (lane group g = lane / 16 holds k = 8g .. 8g+7) with one
+// 16-byte LDS read.  Output layout as v_mfma_f32_16x16x4f32.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+constexpr int TKB = 32;          // K chunk
+constexpr int LDB = TKB + 8;     // u16 per staged row (80 B: rows stay 16-byte aligned)
+
+__device__ __forceinline__ unsigned short bf16_bits(float x) {
+    unsigned u = __float_as_uint(x);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+template <class Tin>
+__device__ void gram_tile_bf16(const GramItem& it, int ti, int tj, const int* __restrict__ pidx,
+                               unsigned short (*As)[LDB], unsigned short (*Bs)[LDB]) {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wi = w >> 1, wj = w & 1;
+    const Tin* X = reinterpret_cast<const Tin*>(it.base);
+    v4 acc[2][2];
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) acc[a][b] = v4{0, 0, 0, 0};
+    const bool mfast = it.ldm == 1;
+    for (int k0 = 0; k0 < it.K; k0 += TKB) {
+#pragma unroll
+        for (int r = 0; r < (TKB * TM) / 256; ++r) {
+            const int e = tid + 256 * r;
+            const int m = mfast ? (e % TM) : (e / TKB);
+            const int k = mfast ? (e / TM) : (e % TKB);
+            const int kg = k0 + k;
+            const int mi = ti * TM + m, mj = tj * TM + m;
+            float za = 0, zb = 0;
+            if (kg < it.K) {
+                if (mi < it.M)
+                    za = (float)((to_d(X[(size_t)kg * it.ldk + (size_t)mi * it.ldm]) - it.mu[mi]) * it.is[mi]);
+                if (mj < it.M)
+                    zb = (float)((to_d(X[(size_t)kg * it.ldk + (size_t)mj * it.ldm]) - it.mu[mj]) * it.is[mj]);
+            }
+            As[m][k] = bf16_bits(za);
+            Bs[m][k] = bf16_bits(zb);
+        }
+        __syncthreads();
+        u16x8 a[2], b[2];
+        for (int q = 0; q < 2; ++q) {
+            a[q] = *reinterpret_cast<const u16x8*>(&As[wi * 32 + q * 16 + (lane & 15)][8 * (lane >> 4)]);
+            b[q] = *reinterpret_cast<const u16x8*>(&Bs[wj * 32 + q * 16 + (lane & 15)][8 * (lane >> 4)]);
+        }
+        for (int qa = 0; qa < 2; ++qa)
+            for (int qb = 0; qb < 2; ++qb)
+                acc[qa][qb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    __builtin_bit_cast(bf16x8, a[qa]), __builtin_bit_cast(bf16x8, b[qb]), acc[qa][qb], 0, 0, 0);
+        __syncthreads();
+    }
+    float* out = reinterpret_cast<float*>(it.out);
+    for (int qa = 0; qa < 2; ++qa)
+        for (int qb = 0; qb < 2; ++qb)
+            for (int r = 0; r < 4; ++r) {
+                const int row = ti * TM + wi * 32 + qa * 16 + 4 * (lane >> 4) + r;
+                const int col = tj * TM + wj * 32 + qb * 16 + (lane & 15);
+                if (row < it.M && col < row) {
+                    const long long p = pidx ? pidx[(size_t)row * it.M + col]
+                                             : (long long)row * (row - 1) / 2 + col;
+                    if (p >= 0) out[p] = acc[qa][qb][r];
+                }
+            }
+}
+
 }  // namespace
+
+template <class Tin>
+__global__ __launch_bounds__(256) void k_gram_fc_bf16(const Tin* __restrict__ bold, int N, int output_ts,
+                                                      int n_seg, const int* __restrict__ seg_start,
+                                                      const int* __restrict__ seg_len,
+                                                      const double* __restrict__ mean,
+                                                      const double* __restrict__ inv_ssd,
+                                                      const int* __restrict__ pidx, int np,
+                                                      float* __restrict__ fc, float* __restrict__ wfc) {
+    __shared__ __align__(16) unsigned short As[TM][LDB];
+    __shared__ __align__(16) unsigned short Bs[TM][LDB];
+    int ti, tj;
+    tile_pair(blockIdx.x, ti, tj);
+    const int seg = blockIdx.y, sim = blockIdx.z;
+    GramItem it;
+    it.base = bold + ((size_t)sim * output_ts + seg_start[seg]) * N;
+    it.K = seg_len[seg];
+    it.M = N;
+    it.ldk = N;
+    it.ldm = 1;
+    const size_t mo = ((size_t)sim * n_seg + seg) * N;
+    it.mu = mean + mo;
+    it.is = inv_ssd + mo;
+    it.out = seg == 0 ? (void*)(fc + (size_t)sim * np)
+                      : (void*)(wfc + ((size_t)sim * (n_seg - 1) + (seg - 1)) * np);
+    gram_tile_bf16<Tin>(it, ti, tj, pidx, As, Bs);
+}
+
+__global__ __launch_bounds__(256) void k_gram_fcd_bf16(const float* __restrict__ wfc, int nw, int np,
+                                                       const double* __restrict__ mean,
+                                                       const double* __restrict__ inv_ssd,
+                                                       float* __restrict__ fcd) {
+    __shared__ __align__(16) unsigned short As[TM][LDB];
+    __shared__ __align__(16) unsigned short Bs[TM][LDB];
+    int ti, tj;
+    tile_pair(blockIdx.x, ti, tj);
+    const int sim = blockIdx.z;
+    GramItem it;
+    it.base = wfc + (size_t)sim * nw * np;
+    it.K = np;
+    it.M = nw;
+    it.ldk = 1;
+    it.ldm = np;
+    it.mu = mean + (size_t)sim * nw;
+    it.is = inv_ssd + (size_t)sim * nw;
+    it.out = fcd + (size_t)sim * ((long long)nw * (nw - 1) / 2);
+    gram_tile_bf16<float>(it, ti, tj, nullptr, As, Bs);
+}
 
 template <class Tin, class Tc, class Tout>
 __global__ __launch_bounds__(256) void k_gram_fc(const Tin* __restrict__ bold, int N, int output_ts,
@@ -279,6 +398,31 @@ hipError_t launch_gram_fcd(const Tin* wfc, int B, int nw, int np, const double* 
     const int nt = (nw + TM - 1) / TM;
     hipLaunchKernelGGL((k_gram_fcd<Tin, Tc, Tout>), dim3(nt * (nt + 1) / 2, 1, B), dim3(256), 0, st,
                        wfc, nw, np, mean, inv_ssd, fcd);
+    return hipGetLastError();
+}
+
+template <class Tin>
+hipError_t launch_gram_fc_bf16(const Tin* bold, int B, int N, int output_ts, int n_seg,
+                               const int* seg_start, const int* seg_len, const double* mean,
+                               const double* inv_ssd, const int* pidx, int np, float* fc, float* wfc,
+                               hipStream_t st) {
+    const int nt = (N + TM - 1) / TM;
+    hipLaunchKernelGGL(k_gram_fc_bf16<Tin>, dim3(nt * (nt + 1) / 2, n_seg, B), dim3(256), 0, st, bold, N,
+                       output_ts, n_seg, seg_start, seg_len, mean, inv_ssd, pidx, np, fc, wfc);
+    return hipGetLastError();
+}
+template hipError_t launch_gram_fc_bf16<float>(const float*, int, int, int, int, const int*, const int*,
+                                               const double*, const double*, const int*, int, float*,
+                                               float*, hipStream_t);
+template hipError_t launch_gram_fc_bf16<double>(const double*, int, int, int, int, const int*, const int*,
+                                                const double*, const double*, const int*, int, float*,
+                                                float*, hipStream_t);
+
+hipError_t launch_gram_fcd_bf16(const float* wfc, int B, int nw, int np, const double* mean,
+                                const double* inv_ssd, float* fcd, hipStream_t st) {
+    const int nt = (nw + TM - 1) / TM;
+    hipLaunchKernelGGL(k_gram_fcd_bf16, dim3(nt * (nt + 1) / 2, 1, B), dim3(256), 0, st, wfc, nw, np, mean,
+                       inv_ssd, fcd);
     return hipGetLastError();
 }
 
