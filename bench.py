@@ -119,6 +119,82 @@ def cpu_baseline(SC, params, noise, N, T_sample, n_sims, threads):
     return wall, integrated
 
 
+def surrogate_experiment(args):
+    """Surrogate-guided CMA-ES (learned FIC initialiser + fit pre-screening)
+    against plain CMA-ES, both through the run_cmaes program on the same
+    synthetic inputs (N=100, 450 s, one z map -> 5 free parameters, FP32):
+    full simulations until the surrogate run's best cost reaches the plain
+    run's final best, FIC restarts per simulation, and the plain re-run of
+    each run's best point."""
+    import csv
+    import subprocess
+    import tempfile
+    from bnm_cuda_amd import inputs as I
+    T, lam, its = args.duration_ms, 128, args.generations + 1
+    work = tempfile.mkdtemp(prefix="bnm_surr_")
+    log(f"[surrogate] writing inputs to {work}")
+    I.write_inputs(work, nodes=100, duration_ms=T, chunked_noise=True)
+    exe = os.path.join(ROOT, "bnm_cuda_amd", "run_cmaes")
+
+    def run(tag, lam_, env_extra):
+        out = os.path.join(work, tag)
+        env = dict(os.environ, BNM_PRECISION="f32", BNM_NOISE_CHUNKED="1", **env_extra)
+        cmd = [exe, os.path.join(work, "SC.txt"), out, os.path.join(work, "emp_FCtril.txt"),
+               os.path.join(work, "emp_FCDtril.txt"), os.path.join(work, "maps.txt"), "100", "0.5-4",
+               "0.05-0.75", "0.05-0.75", "0", "wee-wei", str(T), "1000", "5", "30", "1", str(lam_), str(its),
+               str(args.seed_mw)]
+        log(f"[surrogate] {tag}: {' '.join(cmd[7:])} {env_extra}")
+        t0 = time.perf_counter()
+        r = subprocess.run(cmd, env=env, stdout=sys.stderr, stderr=sys.stderr, timeout=3000)
+        if r.returncode != 0:
+            raise RuntimeError(f"run_cmaes {tag} failed with {r.returncode}")
+        wall = time.perf_counter() - t0
+        cdir = os.path.join(out, "cmaes_multimaps_gpu")
+        prog = [f for f in os.listdir(cdir) if f.endswith("_progress.csv")][0]
+        with open(os.path.join(cdir, prog)) as f:
+            rows = list(csv.DictReader(f))
+        logf = [f for f in os.listdir(cdir) if f.endswith(".txt")][0]
+        text = open(os.path.join(cdir, logf)).read()
+        tail = text.split("Best feasible parameters: ")[1]
+        rerun = -float(tail.split("gof=")[1].split(",")[0])
+        pen = float(tail.split("fic_penalty=")[1].split(",")[0])
+        return rows, wall, rerun + pen, text
+
+    plain, w_plain, plain_rerun, _ = run("plain", lam, {})
+    surr, w_surr, surr_rerun, surr_text = run("surrogate", args.surrogate_lambda,
+                                              {"BNM_SURROGATE": "1", "BNM_SURR_POOL": str(args.pool)})
+    best_plain = float(plain[-1]["best_cost"])
+    sims_plain_total = int(plain[-1]["full_sims"])
+    sims_plain_first = next(int(r["full_sims"]) for r in plain if float(r["best_cost"]) <= best_plain)
+    reach = [int(r["full_sims"]) for r in surr if float(r["best_cost"]) <= best_plain]
+    line = {
+        "metric": "full simulations for surrogate-guided CMA-ES to reach the plain CMA-ES best fit",
+        "value": reach[0] if reach else None, "unit": "full simulations", "higher_is_better": False,
+        "n_gpus": 1, "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"CMA-ES N=100, {T / 1000:.0f} s, 5 free parameters (G, wEE, wEI, "
+                               f"weescale_0, weiscale_0), {args.generations} generations",
+                   "plain_lambda": lam, "surrogate_lambda": args.surrogate_lambda, "pool": args.pool},
+        "plain": {"best_cost": best_plain, "full_sims_total": sims_plain_total,
+                  "full_sims_to_best": sims_plain_first, "wall_s": round(w_plain, 1),
+                  "mean_fic_restarts": float(np.mean([float(r["mean_fic_trials"]) for r in plain])),
+                  "best_rerun_plain_cost": plain_rerun},
+        "surrogate": {"best_cost": float(surr[-1]["best_cost"]), "full_sims_total": int(surr[-1]["full_sims"]),
+                      "wall_s": round(w_surr, 1),
+                      "mean_fic_restarts": float(np.mean([float(r["mean_fic_trials"]) for r in surr])),
+                      "generations_prescreened": int(sum(int(r["surrogate_used"]) for r in surr)),
+                      "rank_corr": [None if r["surrogate_rho"] in ("nan", "") else float(r["surrogate_rho"])
+                                    for r in surr],
+                      "best_rerun_plain_cost": surr_rerun,
+                      "fic_training": surr_text.split("Surrogate FIC initialiser: ")[1].split("\n")[0]
+                      if "Surrogate FIC initialiser: " in surr_text else None},
+        "reduction_vs_plain_to_best": round(sims_plain_first / reach[0], 2) if reach else None,
+        "reduction_vs_plain_total": round(sims_plain_total / reach[0], 2) if reach else None,
+        "progress": {"plain": [(int(r["full_sims"]), float(r["best_cost"])) for r in plain],
+                     "surrogate": [(int(r["full_sims"]), float(r["best_cost"])) for r in surr]},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,6 +205,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-sample-ms", type=int, default=60_000)
     ap.add_argument("--cpu-sims", type=int, default=16)
+    ap.add_argument("--workload", default="pop128", choices=["pop128", "surrogate"],
+                    help="pop128: the headline line; surrogate: CMA-ES with vs without the surrogate")
+    ap.add_argument("--generations", type=int, default=32)
+    ap.add_argument("--surrogate-lambda", type=int, default=64)
+    ap.add_argument("--pool", type=int, default=4)
+    ap.add_argument("--seed-mw", type=int, default=1)
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -137,6 +219,9 @@ def main():
               "with no per-step exchange, but this build measures the one-GPU share.",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "surrogate":
+        surrogate_experiment(args)
+        return
 
     N, B, T = 100, 128, args.duration_ms
     log(f"[bench] generating inputs: N={N} B={B} T={T} ms, noise {(T + 1) * 20 * N / 1e6:.0f} M draws")

@@ -23,7 +23,7 @@ BNM_E_UNSUP = -5
 
 PREC_F64, PREC_F32, PREC_BF16, PREC_BF16X3 = 0, 1, 2, 3
 KERNEL_AUTO, KERNEL_PLAIN, KERNEL_FAST, KERNEL_MFMA = 0, 1, 2, 3
-FC_F64, FC_F32, FC_BF16 = 0, 1, 2
+FC_F64, FC_F32, FC_BF16, FC_BF16X3 = 0, 1, 2, 3
 
 # every symbol include/bnm_hip.h declares
 EXPORTS = (

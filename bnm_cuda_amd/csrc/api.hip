@@ -70,6 +70,7 @@ struct bnm_ctx {
     bool f64 = true;      // integration precision
     bool fc64 = true;     // Gram precision
     bool fcbf16 = false;  // BF16 Gram operands (FP32 accumulation and storage)
+    bool fcsplit = false; // ... as hi/lo pairs (three products)
     hipStream_t st = nullptr;
     double* SC64 = nullptr;
     float* SC32 = nullptr;
@@ -397,13 +398,14 @@ bnm_ctx* bnm_ctx_create(const bnm_config* cfg, int* status) {
     x->B = c.n_sims;
     x->ots = c.time_steps / c.bold_tr + 1;
     x->f64 = c.precision == BNM_PREC_F64;
-    if (c.fc_precision < BNM_FC_F64 || c.fc_precision > BNM_FC_BF16) {
+    if (c.fc_precision < BNM_FC_F64 || c.fc_precision > BNM_FC_BF16X3) {
         set(fail(BNM_E_ARG, "unknown fc_precision"));
         delete x;
         return nullptr;
     }
     x->fc64 = c.fc_precision == BNM_FC_F64;
-    x->fcbf16 = c.fc_precision == BNM_FC_BF16;
+    x->fcbf16 = c.fc_precision == BNM_FC_BF16 || c.fc_precision == BNM_FC_BF16X3;
+    x->fcsplit = c.fc_precision == BNM_FC_BF16X3;
     if (!c.sim_only) {
         x->np = bnm_n_pairs(c.nodes, c.exc_interhemispheric);
         x->nw = c.no_fcd ? 0 : bnm_n_windows(&c);
@@ -705,7 +707,7 @@ static int enqueue_run(bnm_ctx* x, hipStream_t st, bool timed) {
             else if (x->fcbf16)
                 HIPCHK(launch_gram_fc_bf16<double>(
                     (const double*)x->bold, x->B, x->N, x->ots, x->nseg, x->seg_start, x->seg_len,
-                    x->mean, x->inv, x->pidx, x->np, (float*)x->fc, (float*)x->wfc, st));
+                    x->mean, x->inv, x->pidx, x->np, (float*)x->fc, (float*)x->wfc, x->fcsplit, st));
             else
                 HIPCHK((launch_gram_fc<double, float, float>(
                     (const double*)x->bold, x->B, x->N, x->ots, x->nseg, x->seg_start, x->seg_len,
@@ -720,7 +722,7 @@ static int enqueue_run(bnm_ctx* x, hipStream_t st, bool timed) {
             else if (x->fcbf16)
                 HIPCHK(launch_gram_fc_bf16<float>(
                     (const float*)x->bold, x->B, x->N, x->ots, x->nseg, x->seg_start, x->seg_len,
-                    x->mean, x->inv, x->pidx, x->np, (float*)x->fc, (float*)x->wfc, st));
+                    x->mean, x->inv, x->pidx, x->np, (float*)x->fc, (float*)x->wfc, x->fcsplit, st));
             else
                 HIPCHK((launch_gram_fc<float, float, float>(
                     (const float*)x->bold, x->B, x->N, x->ots, x->nseg, x->seg_start, x->seg_len,
@@ -763,7 +765,7 @@ static int enqueue_run(bnm_ctx* x, hipStream_t st, bool timed) {
                                              x->wmean, x->winv, st));
             if (x->fcbf16)
                 HIPCHK(launch_gram_fcd_bf16((const float*)x->wfc, x->B, x->nw, x->np, x->wmean, x->winv,
-                                            (float*)x->fcd, st));
+                                            (float*)x->fcd, x->fcsplit, st));
             else
                 HIPCHK((launch_gram_fcd<float, float, float>((const float*)x->wfc, x->B, x->nw, x->np,
                                                              x->wmean, x->winv, (float*)x->fcd, st)));

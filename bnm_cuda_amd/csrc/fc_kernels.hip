@@ -208,7 +208,11 @@ __device__ __forceinline__ unsigned short bf16_bits(float x) {
     return (unsigned short)(u >> 16);
 }
 
-template <class Tin>
+// SPLIT: each z-score is carried as hi = bf16(z) and lo = bf16(z - hi), and
+// the product as hi*hi + hi*lo + lo*hi (three MFMAs; the dropped lo*lo term
+// is 2^-16 relative), which brings the Gram to FP32-like accuracy.  The
+// As/Bs arrays then hold [hi rows | lo rows] (2*TM rows).
+template <class Tin, bool SPLIT>
 __device__ void gram_tile_bf16(const GramItem& it, int ti, int tj, const int* __restrict__ pidx,
                                unsigned short (*As)[LDB], unsigned short (*Bs)[LDB]) {
     typedef float v4 __attribute__((ext_vector_type(4)));
@@ -234,19 +238,36 @@ __device__ void gram_tile_bf16(const GramItem& it, int ti, int tj, const int* __
                 if (mj < it.M)
                     zb = (float)((to_d(X[(size_t)kg * it.ldk + (size_t)mj * it.ldm]) - it.mu[mj]) * it.is[mj]);
             }
-            As[m][k] = bf16_bits(za);
-            Bs[m][k] = bf16_bits(zb);
+            const unsigned short ha = bf16_bits(za), hb = bf16_bits(zb);
+            As[m][k] = ha;
+            Bs[m][k] = hb;
+            if constexpr (SPLIT) {
+                As[TM + m][k] = bf16_bits(za - __uint_as_float((unsigned)ha << 16));
+                Bs[TM + m][k] = bf16_bits(zb - __uint_as_float((unsigned)hb << 16));
+            }
         }
         __syncthreads();
-        u16x8 a[2], b[2];
+        u16x8 a[2], b[2], al[2], bl[2];
         for (int q = 0; q < 2; ++q) {
-            a[q] = *reinterpret_cast<const u16x8*>(&As[wi * 32 + q * 16 + (lane & 15)][8 * (lane >> 4)]);
-            b[q] = *reinterpret_cast<const u16x8*>(&Bs[wj * 32 + q * 16 + (lane & 15)][8 * (lane >> 4)]);
+            const int ra = wi * 32 + q * 16 + (lane & 15), rb = wj * 32 + q * 16 + (lane & 15);
+            a[q] = *reinterpret_cast<const u16x8*>(&As[ra][8 * (lane >> 4)]);
+            b[q] = *reinterpret_cast<const u16x8*>(&Bs[rb][8 * (lane >> 4)]);
+            if constexpr (SPLIT) {
+                al[q] = *reinterpret_cast<const u16x8*>(&As[TM + ra][8 * (lane >> 4)]);
+                bl[q] = *reinterpret_cast<const u16x8*>(&Bs[TM + rb][8 * (lane >> 4)]);
+            }
         }
         for (int qa = 0; qa < 2; ++qa)
-            for (int qb = 0; qb < 2; ++qb)
+            for (int qb = 0; qb < 2; ++qb) {
                 acc[qa][qb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
                     __builtin_bit_cast(bf16x8, a[qa]), __builtin_bit_cast(bf16x8, b[qb]), acc[qa][qb], 0, 0, 0);
+                if constexpr (SPLIT) {
+                    acc[qa][qb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        __builtin_bit_cast(bf16x8, a[qa]), __builtin_bit_cast(bf16x8, bl[qb]), acc[qa][qb], 0, 0, 0);
+                    acc[qa][qb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        __builtin_bit_cast(bf16x8, al[qa]), __builtin_bit_cast(bf16x8, b[qb]), acc[qa][qb], 0, 0, 0);
+                }
+            }
         __syncthreads();
     }
     float* out = reinterpret_cast<float*>(it.out);
@@ -265,7 +286,7 @@ __device__ void gram_tile_bf16(const GramItem& it, int ti, int tj, const int* __
 
 }  // namespace
 
-template <class Tin>
+template <class Tin, bool SPLIT>
 __global__ __launch_bounds__(256) void k_gram_fc_bf16(const Tin* __restrict__ bold, int N, int output_ts,
                                                       int n_seg, const int* __restrict__ seg_start,
                                                       const int* __restrict__ seg_len,
@@ -273,8 +294,8 @@ __global__ __launch_bounds__(256) void k_gram_fc_bf16(const Tin* __restrict__ bo
                                                       const double* __restrict__ inv_ssd,
                                                       const int* __restrict__ pidx, int np,
                                                       float* __restrict__ fc, float* __restrict__ wfc) {
-    __shared__ __align__(16) unsigned short As[TM][LDB];
-    __shared__ __align__(16) unsigned short Bs[TM][LDB];
+    __shared__ __align__(16) unsigned short As[SPLIT ? 2 * TM : TM][LDB];
+    __shared__ __align__(16) unsigned short Bs[SPLIT ? 2 * TM : TM][LDB];
     int ti, tj;
     tile_pair(blockIdx.x, ti, tj);
     const int seg = blockIdx.y, sim = blockIdx.z;
@@ -289,15 +310,16 @@ __global__ __launch_bounds__(256) void k_gram_fc_bf16(const Tin* __restrict__ bo
     it.is = inv_ssd + mo;
     it.out = seg == 0 ? (void*)(fc + (size_t)sim * np)
                       : (void*)(wfc + ((size_t)sim * (n_seg - 1) + (seg - 1)) * np);
-    gram_tile_bf16<Tin>(it, ti, tj, pidx, As, Bs);
+    gram_tile_bf16<Tin, SPLIT>(it, ti, tj, pidx, As, Bs);
 }
 
+template <bool SPLIT>
 __global__ __launch_bounds__(256) void k_gram_fcd_bf16(const float* __restrict__ wfc, int nw, int np,
                                                        const double* __restrict__ mean,
                                                        const double* __restrict__ inv_ssd,
                                                        float* __restrict__ fcd) {
-    __shared__ __align__(16) unsigned short As[TM][LDB];
-    __shared__ __align__(16) unsigned short Bs[TM][LDB];
+    __shared__ __align__(16) unsigned short As[SPLIT ? 2 * TM : TM][LDB];
+    __shared__ __align__(16) unsigned short Bs[SPLIT ? 2 * TM : TM][LDB];
     int ti, tj;
     tile_pair(blockIdx.x, ti, tj);
     const int sim = blockIdx.z;
@@ -310,7 +332,7 @@ __global__ __launch_bounds__(256) void k_gram_fcd_bf16(const float* __restrict__
     it.mu = mean + (size_t)sim * nw;
     it.is = inv_ssd + (size_t)sim * nw;
     it.out = fcd + (size_t)sim * ((long long)nw * (nw - 1) / 2);
-    gram_tile_bf16<float>(it, ti, tj, nullptr, As, Bs);
+    gram_tile_bf16<float, SPLIT>(it, ti, tj, nullptr, As, Bs);
 }
 
 template <class Tin, class Tc, class Tout>
@@ -405,24 +427,32 @@ template <class Tin>
 hipError_t launch_gram_fc_bf16(const Tin* bold, int B, int N, int output_ts, int n_seg,
                                const int* seg_start, const int* seg_len, const double* mean,
                                const double* inv_ssd, const int* pidx, int np, float* fc, float* wfc,
-                               hipStream_t st) {
+                               bool split, hipStream_t st) {
     const int nt = (N + TM - 1) / TM;
-    hipLaunchKernelGGL(k_gram_fc_bf16<Tin>, dim3(nt * (nt + 1) / 2, n_seg, B), dim3(256), 0, st, bold, N,
-                       output_ts, n_seg, seg_start, seg_len, mean, inv_ssd, pidx, np, fc, wfc);
+    const dim3 grid(nt * (nt + 1) / 2, n_seg, B);
+    if (split)
+        hipLaunchKernelGGL((k_gram_fc_bf16<Tin, true>), grid, dim3(256), 0, st, bold, N, output_ts, n_seg,
+                           seg_start, seg_len, mean, inv_ssd, pidx, np, fc, wfc);
+    else
+        hipLaunchKernelGGL((k_gram_fc_bf16<Tin, false>), grid, dim3(256), 0, st, bold, N, output_ts, n_seg,
+                           seg_start, seg_len, mean, inv_ssd, pidx, np, fc, wfc);
     return hipGetLastError();
 }
 template hipError_t launch_gram_fc_bf16<float>(const float*, int, int, int, int, const int*, const int*,
                                                const double*, const double*, const int*, int, float*,
-                                               float*, hipStream_t);
+                                               float*, bool, hipStream_t);
 template hipError_t launch_gram_fc_bf16<double>(const double*, int, int, int, int, const int*, const int*,
                                                 const double*, const double*, const int*, int, float*,
-                                                float*, hipStream_t);
+                                                float*, bool, hipStream_t);
 
 hipError_t launch_gram_fcd_bf16(const float* wfc, int B, int nw, int np, const double* mean,
-                                const double* inv_ssd, float* fcd, hipStream_t st) {
+                                const double* inv_ssd, float* fcd, bool split, hipStream_t st) {
     const int nt = (nw + TM - 1) / TM;
-    hipLaunchKernelGGL(k_gram_fcd_bf16, dim3(nt * (nt + 1) / 2, 1, B), dim3(256), 0, st, wfc, nw, np, mean,
-                       inv_ssd, fcd);
+    const dim3 grid(nt * (nt + 1) / 2, 1, B);
+    if (split)
+        hipLaunchKernelGGL(k_gram_fcd_bf16<true>, grid, dim3(256), 0, st, wfc, nw, np, mean, inv_ssd, fcd);
+    else
+        hipLaunchKernelGGL(k_gram_fcd_bf16<false>, grid, dim3(256), 0, st, wfc, nw, np, mean, inv_ssd, fcd);
     return hipGetLastError();
 }
 

@@ -68,14 +68,15 @@ hipError_t launch_gram_fc(const Tin* bold, int B, int N, int output_ts, int n_se
 template <class Tin, class Tc, class Tout>
 hipError_t launch_gram_fcd(const Tin* wfc, int B, int nw, int np, const double* mean,
                            const double* inv_ssd, Tout* fcd, hipStream_t st);
-// BF16-operand variants (FP32 accumulation, FP32 outputs)
+// BF16-operand variants (FP32 accumulation, FP32 outputs); split = hi/lo
+// three-product form
 template <class Tin>
 hipError_t launch_gram_fc_bf16(const Tin* bold, int B, int N, int output_ts, int n_seg,
                                const int* seg_start, const int* seg_len, const double* mean,
                                const double* inv_ssd, const int* pidx, int np, float* fc, float* wfc,
-                               hipStream_t st);
+                               bool split, hipStream_t st);
 hipError_t launch_gram_fcd_bf16(const float* wfc, int B, int nw, int np, const double* mean,
-                                const double* inv_ssd, float* fcd, hipStream_t st);
+                                const double* inv_ssd, float* fcd, bool split, hipStream_t st);
 // Row moments of the window-FC matrix (mean, 1/ssd over the np edges).
 template <class T>
 hipError_t launch_row_moments(const T* x, long long rows, int len, double* mean, double* inv_ssd,

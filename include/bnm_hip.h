@@ -55,7 +55,8 @@ extern "C" {
 /* FC / FCD Gram precision */
 #define BNM_FC_F64 0
 #define BNM_FC_F32 1
-#define BNM_FC_BF16 2
+#define BNM_FC_BF16 2       /* BF16 operands, FP32 accumulation              */
+#define BNM_FC_BF16X3 3     /* hi/lo BF16 split operands, three products     */
 
 typedef struct bnm_config {
     int nodes;                 /* N                                               */

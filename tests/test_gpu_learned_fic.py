@@ -49,6 +49,7 @@ def test_learned_start_cuts_fic_restarts():
     d_l = np.abs(start - w_a)[ok].mean()
     print(f"restarts analytical {t_a.mean():.3f} (max {t_a.max()}, failed {f_a.sum()}) | "
           f"learned {t_l.mean():.3f} (max {t_l.max()}, failed {f_l.sum()}) | "
+          f"both converged ({ok.sum()}): {t_a[ok].mean():.3f} -> {t_l[ok].mean():.3f} | "
           f"|start - converged| {d_an:.4f} -> {d_l:.4f}")
     model.close()
     assert t_a[ok].mean() > 1.0              # the analytical start needs restarts here
