@@ -133,7 +133,7 @@ def surrogate_experiment(args):
     T, lam, its = args.duration_ms, 128, args.generations + 1
     work = tempfile.mkdtemp(prefix="bnm_surr_")
     log(f"[surrogate] writing inputs to {work}")
-    I.write_inputs(work, nodes=100, duration_ms=T, chunked_noise=True)
+    I.write_inputs(work, nodes=100, duration_ms=T, window=args.window, step=args.step, chunked_noise=True)
     exe = os.path.join(ROOT, "bnm_cuda_amd", "run_cmaes")
 
     def run(tag, lam_, env_extra):
@@ -141,7 +141,8 @@ def surrogate_experiment(args):
         env = dict(os.environ, BNM_PRECISION="f32", BNM_NOISE_CHUNKED="1", **env_extra)
         cmd = [exe, os.path.join(work, "SC.txt"), out, os.path.join(work, "emp_FCtril.txt"),
                os.path.join(work, "emp_FCDtril.txt"), os.path.join(work, "maps.txt"), "100", "0.5-4",
-               "0.05-0.75", "0.05-0.75", "0", "wee-wei", str(T), "1000", "5", "30", "1", str(lam_), str(its),
+               "0.05-0.75", "0.05-0.75", "0", "wee-wei", str(T), "1000", str(args.step), str(args.window), "1",
+               str(lam_), str(its),
                str(args.seed_mw)]
         log(f"[surrogate] {tag}: {' '.join(cmd[7:])} {env_extra}")
         t0 = time.perf_counter()
@@ -211,6 +212,8 @@ def main():
     ap.add_argument("--surrogate-lambda", type=int, default=64)
     ap.add_argument("--pool", type=int, default=4)
     ap.add_argument("--seed-mw", type=int, default=1)
+    ap.add_argument("--window", type=int, default=30, help="FCD window (surrogate workload)")
+    ap.add_argument("--step", type=int, default=5, help="FCD window step (surrogate workload)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
