@@ -59,7 +59,8 @@ int dalloc(T** p, size_t n) {
 
 // One launchable integration kernel shape.
 struct Variant {
-    int kind;   // 0 split (SC in registers), 1 stream (SC from L2), 2 mfma, 3 plain
+    int kind;   // 0 split (SC in registers), 1 stream (SC from L2), 2 mfma, 3 plain,
+                // 4 split with the E/I pools on separate lanes (FP32)
     int ls, ks, mode;
 };
 
@@ -100,7 +101,7 @@ struct bnm_ctx {
     Variant variant{0, 4, 1, 0};
     hipGraphExec_t graph_exec = nullptr;
     bool have_sc = false, have_noise = false, have_emp = false, have_params = false;
-    int fast_ls = 0, fast_ks = 0, stream_ks = 0, force_stream = 0;
+    int fast_ls = 0, fast_ks = 0, stream_ks = 0, force_stream = 0, no_ei = 0, ei_ls = 0;
 };
 
 extern "C" {
@@ -416,6 +417,8 @@ bnm_ctx* bnm_ctx_create(const bnm_config* cfg, int* status) {
     x->fast_ks = env_int("BNM_FAST_KS", 0);
     x->stream_ks = env_int("BNM_STREAM_KS", 0);
     x->force_stream = env_int("BNM_FORCE_STREAM", 0);
+    x->no_ei = env_int("BNM_NO_EI", 0);
+    x->ei_ls = env_int("BNM_EI_LS", 0);
     x->autotune = c.autotune;
     x->tune_ms = env_int("BNM_TUNE_MS", 300);
     if (hipStreamCreateWithFlags(&x->st, hipStreamNonBlocking) != hipSuccess) {
@@ -535,6 +538,7 @@ static bool variant_ok(const bnm_ctx* x, const Variant& v) {
         case 0: return fast_shape(x->N, v.ls, v.ks, el).ok;
         case 1: return stream_shape(x->N, v.ks, el).ok;
         case 2: return !x->f64 && mfma_shape(x->N, v.mode).ok;
+        case 4: return !x->f64 && ei_shape(x->N, v.ls).ok;
         default: return true;
     }
 }
@@ -556,6 +560,10 @@ static std::string variant_name(const bnm_ctx* x, const Variant& v) {
         const MfmaShape s = mfma_shape(x->N, v.mode);
         std::snprintf(name, sizeof name, "mfma_%s(sims/block=16,waves=%d,KQ=%d,threads=%d)",
                       mn[v.mode], s.waves, s.KQ, s.threads);
+    } else if (v.kind == 4) {
+        const FastShape s = ei_shape(x->N, v.ls);
+        std::snprintf(name, sizeof name, "split_ei_f32(LS=%d,NW=%d,KC=%d,SS=%d,threads=%d)", s.LS, s.NW, s.KC,
+                      s.SS, s.threads);
     } else {
         std::snprintf(name, sizeof name, "plain_%s(threads=%d)", p, ((x->N + 63) / 64) * 64);
     }
@@ -571,6 +579,7 @@ static hipError_t launch_variant(bnm_ctx* x, const Variant& v, const SimArgs& a,
             return x->f64 ? launch_sim_stream<double>(x->SCt64, a, v.ks, st)
                           : launch_sim_stream<float>(x->SCt32, a, v.ks, st);
         case 2: return launch_sim_mfma(x->SC32, a, v.mode, st);
+        case 4: return launch_sim_ei(x->SC32, a, v.ls, st);
         default:
             return x->f64 ? launch_sim_plain<double>(x->SC64, a, st)
                           : launch_sim_plain<float>(x->SC32, a, st);
@@ -595,12 +604,20 @@ static std::vector<Variant> candidates(const bnm_ctx* x) {
         add({2, 0, 0, 0});
         return out;
     }
+    if (x->ei_ls > 0 && !x->f64) {
+        add({4, x->ei_ls, 1, 0});
+        if (!out.empty()) return out;
+    }
     const bool pinned = x->fast_ls > 0 || x->fast_ks > 0;
     if (pinned && !x->force_stream) {
         add({0, x->fast_ls > 0 ? x->fast_ls : 4, x->fast_ks > 0 ? x->fast_ks : 1, 0});
         if (!out.empty()) return out;
     }
     if (!x->force_stream) {
+        if (!x->f64 && !x->no_ei) {
+            add({4, 4, 1, 0});
+            add({4, 2, 1, 0});
+        }
         static const int pref[][2] = {{4, 1}, {2, 1}, {1, 1}, {2, 2}, {1, 2}, {4, 2},
                                       {1, 4}, {2, 4}, {1, 8}, {2, 8}, {1, 16}};
         for (const auto& p : pref) add({0, p[0], p[1], 0});

@@ -27,6 +27,9 @@ template <class R>
 hipError_t launch_sim_fast(const R* SC, const SimArgs& a, int LS, int KS, hipStream_t st);
 // large N: SC streamed from a transposed copy SCt[k][j] in global memory
 FastShape stream_shape(int N, int KS, size_t elem);
+// FP32 split kernel with the node update divided between lanes by population
+FastShape ei_shape(int N, int LS);
+hipError_t launch_sim_ei(const float* SC, const SimArgs& a, int LS, hipStream_t st);
 template <class R>
 hipError_t launch_sim_stream(const R* SCt, const SimArgs& a, int KS, hipStream_t st);
 

@@ -316,6 +316,190 @@ __global__ __launch_bounds__(MAXT) void k_sim_split(const R* __restrict__ SC, Si
 }
 
 // ---------------------------------------------------------------------------
+// split_ei (FP32): the split kernel with the node update divided between the
+// lanes of a node by population.  After the coupling reduction every lane of
+// node j holds c_j; lane slice 0 advances the excitatory pool (S_E, I_E, r_E,
+// BW, FIC) and slice 1 the inhibitory pool (S_I, I_I, r_I), running the same
+// instruction stream on lane-specific constants:
+//   I  = k0 + kSE*S_E + kc*c - kSI*S_I      E: wE*I0, wEE, G*J_NMDA, wIE
+//                                           I: wI*I0, wEI, 0,        1
+//   r  = (aI - b) / (1 - exp(-d (aI - b)))  (a, b, d of the pool)
+//   dS = xi*sigma*sqrt(dt) + (g*r - g*r*S*m - S*dt/tau)   m = 1 (E), 0 (I)
+// Each lane swaps S with its partner (v_permlane16/32_swap) before the
+// update, so both see the previous substep's S_E and S_I.  A wave issues
+// one pool's update instead of both, which is what bounds the split kernel
+// (two waves per SIMD at N = 100).  LS = 2 or 4, KS = 1; slices 2 and 3 of
+// LS = 4 mirror slices 0 and 1.
+// ---------------------------------------------------------------------------
+template <int KC, int LS, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_sim_ei(const float* __restrict__ SC, SimArgs a, int NW, int SS) {
+    static_assert(LS == 2 || LS == 4, "E/I split needs two lanes per node");
+    static_assert(KC % 4 == 0, "slice length must be a multiple of 4");
+    constexpr int GN = 64 / LS;
+    const int sim = blockIdx.x;
+    const int N = a.N;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ln = lane % GN, sl = lane / GN;
+    const int j = wave * GN + ln;
+    const int pop = sl & 1;               // 0: excitatory pool, 1: inhibitory
+    const int k0i = sl * KC;
+    const bool comp = j < N;
+    const bool writer = comp && sl == 0;  // publishes S_E, BOLD, wIE
+    const bool owner = comp && sl < 2;    // writes its pool's means
+    const int ldS = LS * SS;
+    const int posj = (j / KC) * SS + (j % KC);
+    extern __shared__ unsigned char smem_raw[];
+    float* Sb = reinterpret_cast<float*>(smem_raw);   // [2][ldS]
+
+    float sc[KC];
+#pragma unroll
+    for (int q = 0; q < KC; ++q) {
+        const int k = k0i + q;
+        sc[q] = (j < N && k < N) ? SC[(size_t)j * N + k] : 0.0f;
+    }
+    for (int i = threadIdx.x; i < 2 * ldS; i += blockDim.x) {
+        const int slot = i % ldS, qq = slot % SS, k = (slot / SS) * KC + qq;
+        Sb[i] = (qq < KC && k < N) ? 0.001f : 0.0f;
+    }
+    // pool constants
+    const size_t o = (size_t)sim * N + (comp ? j : 0);
+    const float GJ = gain<float>(a.G[sim]);
+    const float k0 = pop ? (float)C64::wI_I0 : (float)C64::wE_I0;
+    const float kSE = comp ? (float)(pop ? a.wEI[o] : a.wEE[o]) : 0.0f;
+    const float kc = pop ? 0.0f : GJ;
+    const float aa = pop ? (float)C64::a_I : (float)C64::a_E;
+    const float bb = pop ? (float)C64::b_I : (float)C64::b_E;
+    const float ndl = pop ? (float)(-C64::d_I * kLog2e) : (float)(-C64::d_E * kLog2e);
+    const float invd = pop ? (float)(1.0 / C64::d_I) : (float)(1.0 / C64::d_E);
+    const float d12 = pop ? (float)(C64::d_I / 12.0) : (float)(C64::d_E / 12.0);
+    const float g = pop ? (float)(C64::dt * C64::gamma_I) : (float)(C64::dt * C64::gamma_E);
+    const float gm = pop ? 0.0f : g;
+    const float it = pop ? (float)(C64::dt / C64::tau_I) : (float)(C64::dt / C64::tau_E);
+    constexpr float nzs = (float)(C64::sigma * C64::sqrt_dt);
+    float wIE = comp ? (float)a.wIE_in[o] : 0.0f, delta = (float)a.init_delta;
+    float S = 0.001f, I = 0.0f, r = 0.0f;
+    BW<float> bw;
+    double mI = 0, ext[3] = {0, 0, 0};
+
+    int ts = 0, vol = 0, trial = 0, buf = 0;
+    bool adjusting = a.adjust_fic != 0, failed = false;
+    const unsigned dur = (unsigned)(a.i_end - a.i_start + 1);
+    const int P = (int)a.noise_period;
+    const float* noise1 = a.noise + 2 * (size_t)(comp ? j : 0) + pop;   // this pool's draw
+    float nz[10], nzn[10];
+    int ms = 0;
+#pragma unroll
+    for (int s = 0; s < 10; ++s) nz[s] = comp ? noise1[(size_t)s * 2 * N] : 0.0f;
+    __syncthreads();
+    while (ts <= a.time_steps) {
+        const int ms_next = ms + 1 == P ? 0 : ms + 1;
+        if (comp) {
+            const float* p = noise1 + (size_t)ms_next * 20 * N;
+#pragma unroll
+            for (int s = 0; s < 10; ++s) nzn[s] = p[(size_t)s * 2 * N];
+        }
+        const float kSI = pop ? 1.0f : wIE;
+#pragma unroll
+        for (int s = 0; s < 10; ++s) {
+            const float4* Sr = reinterpret_cast<const float4*>(Sb + buf * ldS + sl * SS);
+            float4 sv[KC / 4];
+#pragma unroll
+            for (int q = 0; q < KC / 4; ++q) sv[q] = Sr[q];
+            // previous-substep S of the partner pool, and the part of I that
+            // does not wait for the coupling
+            float So;
+            {
+                auto w = LS == 4 ? __builtin_amdgcn_permlane16_swap(__float_as_uint(S), __float_as_uint(S), false, false)
+                                 : __builtin_amdgcn_permlane32_swap(__float_as_uint(S), __float_as_uint(S), false, false);
+                // element 0: lower-row lane's value, element 1: upper-row lane's
+                So = __uint_as_float(pop ? w[0] : w[1]);
+            }
+            const float SEv = pop ? So : S, SIv = pop ? S : So;
+            const float pre = k0 + kSE * SEv - kSI * SIv;
+            float acc[4] = {0, 0, 0, 0};
+#pragma unroll
+            for (int q = 0; q < KC / 4; ++q) {
+                acc[0] = fmaf(sc[4 * q + 0], sv[q].x, acc[0]);
+                acc[1] = fmaf(sc[4 * q + 1], sv[q].y, acc[1]);
+                acc[2] = fmaf(sc[4 * q + 2], sv[q].z, acc[2]);
+                acc[3] = fmaf(sc[4 * q + 3], sv[q].w, acc[3]);
+            }
+            float c = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+            if constexpr (LS == 4) c = lane_sum16<float>(c);
+            c = lane_sum32<float>(c);
+            I = pre + kc * c;
+            const float x = aa * I - bb;
+            const float e = __builtin_amdgcn_exp2f(ndl * x);
+            const float direct = x * __builtin_amdgcn_rcpf(1.0f - e);
+            const float series = invd + x * (0.5f + d12 * x);
+            r = fabsf(x) < 0.1f ? series : direct;
+            const float dS = nz[s] * nzs + (g * r - gm * r * S - it * S);
+            S = __builtin_amdgcn_fmed3f(S + dS, 0.0f, 1.0f);
+            if (writer) Sb[(buf ^ 1) * ldS + posj] = S;
+            __syncthreads();
+            buf ^= 1;
+        }
+        if (comp && pop == 0) bw.step(S);
+        if (ts % a.bold_tr == 0) {
+            if (writer) {
+                float* bold = reinterpret_cast<float*>(a.bold);
+                bold[((size_t)sim * a.output_ts + vol) * N + j] = bw.bold();
+            }
+            if (vol >= a.n_vols_remove) {
+                ext[0] += (double)S;
+                ext[1] += (double)I;
+                ext[2] += (double)r;
+            }
+            ++vol;
+        }
+        ++ts;
+        bool restarted = false;
+        if (adjusting) {
+            if (ts >= a.i_start && ts <= a.i_end) mI += (double)I;
+            if (ts == a.i_end) {
+                // the excitatory lanes of a node decide alike; inhibitory lanes never do
+                const bool out = comp && pop == 0 && fic_decide<float>(mI, dur, trial, a.max_fic_trials, wIE, delta);
+                if (__syncthreads_or(out)) {
+                    if (trial < a.max_fic_trials) {
+                        S = 0.001f;
+                        bw.reset();
+                        mI = 0;
+                        ext[0] = ext[1] = ext[2] = 0;
+                        if (writer) Sb[buf * ldS + posj] = 0.001f;
+                        ts = 0; vol = 0; ++trial;
+                        restarted = true;
+                        __syncthreads();
+                    } else {
+                        failed = true; adjusting = false;
+                    }
+                } else {
+                    adjusting = false;
+                }
+            }
+        }
+        if (restarted) {
+            ms = 0;
+#pragma unroll
+            for (int s = 0; s < 10; ++s) nz[s] = comp ? noise1[(size_t)s * 2 * N] : 0.0f;
+        } else {
+            ms = ms_next;
+#pragma unroll
+            for (int s = 0; s < 10; ++s) nz[s] = nzn[s];
+        }
+    }
+    if (owner) {
+        if (pop == 0) a.wIE_out[o] = (double)wIE;
+        const int n = vol - a.n_vols_remove;
+        for (int i = 0; i < 3; ++i)
+            a.ext[((size_t)sim * 6 + 3 * pop + i) * N + j] = n > 0 ? ext[i] / n : 0.0;
+    }
+    if (threadIdx.x == 0) {
+        a.fic_failed[sim] = failed ? 1 : 0;
+        a.fic_trials[sim] = trial;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // split_g: the split kernel for node counts whose SC does not fit in
 // registers (N > 256).  Lane j streams its SC column slice from a transposed
 // copy SCt[k][j] in global memory (L2-resident; consecutive lanes read
@@ -505,6 +689,40 @@ hipError_t launch_sim_fast(const R* SC, const SimArgs& a, int LS, int KS, hipStr
         case 64: launch_split_kc<R, 64>(SC, a, s, st); break;
         case 100: launch_split_kc<R, 100>(SC, a, s, st); break;
         default: launch_split_kc<R, 128>(SC, a, s, st); break;
+    }
+    return hipGetLastError();
+}
+
+template <int KC, int LS>
+static void launch_ei(const float* SC, const SimArgs& a, const FastShape& s, hipStream_t st) {
+    constexpr int MAXT = maxt_for(KC, 4);
+    hipLaunchKernelGGL((k_sim_ei<KC, LS, MAXT>), dim3(a.B), dim3(s.threads), s.smem, st, SC, a, s.NW, s.SS);
+}
+
+template <int KC>
+static void launch_ei_kc(const float* SC, const SimArgs& a, const FastShape& s, hipStream_t st) {
+    if (s.LS == 2) launch_ei<KC, 2>(SC, a, s, st);
+    else launch_ei<KC, 4>(SC, a, s, st);
+}
+
+FastShape ei_shape(int N, int LS) {
+    FastShape s = fast_shape(N, LS, 1, sizeof(float));
+    s.ok = s.ok && (LS == 2 || LS == 4);
+    return s;
+}
+
+hipError_t launch_sim_ei(const float* SC, const SimArgs& a, int LS, hipStream_t st) {
+    const FastShape s = ei_shape(a.N, LS);
+    if (!s.ok) return hipErrorInvalidValue;
+    switch (s.KC) {
+        case 8: launch_ei_kc<8>(SC, a, s, st); break;
+        case 16: launch_ei_kc<16>(SC, a, s, st); break;
+        case 28: launch_ei_kc<28>(SC, a, s, st); break;
+        case 32: launch_ei_kc<32>(SC, a, s, st); break;
+        case 52: launch_ei_kc<52>(SC, a, s, st); break;
+        case 64: launch_ei_kc<64>(SC, a, s, st); break;
+        case 100: launch_ei_kc<100>(SC, a, s, st); break;
+        default: launch_ei_kc<128>(SC, a, s, st); break;
     }
     return hipGetLastError();
 }

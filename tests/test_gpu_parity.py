@@ -166,6 +166,42 @@ def test_f32_summary_stats_within_bounds(oracle_mod, case, ref, kernel):
     assert d_ks <= 0.01
 
 
+@pytest.mark.parametrize("ls", [2, 4])
+def test_f32_ei_split_kernel_within_bounds(oracle_mod, case, ref, ls, monkeypatch):
+    """The FP32 kernel that advances the E and I pools on separate lanes
+    (partner state exchanged by lane swaps) against the FP64 oracle: the same
+    FIC trials, summary statistics within bounds, and early BOLD close to the
+    FP64 trajectory (a wrong partner lane or pool constant would not be)."""
+    monkeypatch.setenv("BNM_EI_LS", str(ls))
+    cfg = _cfg(case, bnm.PREC_F32, bnm.KERNEL_FAST)
+    with bnm.Context(cfg) as c:
+        c.set_sc(case["SC"])
+        c.set_noise(case["noise"])
+        c.set_empirical(case["emp_fc"], case["emp_fcd"])
+        c.set_params(case["G"], case["wEE"], case["wEI"])
+        c.run()
+        name = c.kernel_name()
+        g = c.outputs()
+    print("kernel", name)
+    assert name.startswith("split_ei_f32(LS=%d" % ls)
+    O = oracle_mod
+    d_re = np.max(np.abs(g["ext"][:, 2] - ref["ext"][:, 2]))
+    d_ext = np.max(np.abs(g["ext"] - ref["ext"]), axis=(0, 2))
+    d_fc = np.max(np.abs(g["fc"] - ref["fc"]))
+    d_ks = max(abs(O.ks(case["emp_fcd"], g["fcd"][b]) - O.ks(case["emp_fcd"], ref["fcd"][b]))
+               for b in range(B))
+    early = min(g["bold"].shape[1], 11)
+    rel = _bold_rel(g["bold"][:, :early], ref["bold"][:, :early])
+    print(f"LS={ls}: trials {g['fic_trials']} vs {ref['fic_trials']}; d mean rE {d_re:.3g} Hz, "
+          f"ext {d_ext}, d FC {d_fc:.3g}, d KS {d_ks:.3g}, BOLD rel (first 10 s) {rel:.3g}")
+    np.testing.assert_array_equal(g["fic_trials"], ref["fic_trials"])
+    np.testing.assert_allclose(g["wIE_fic"], ref["wIE_fic"], atol=1e-5)
+    assert d_re <= 0.05 and d_fc <= 1e-3 and d_ks <= 0.01
+    assert rel <= 1e-4
+    # every pool's means land in their own slots (S_E, I_E, r_E, S_I, I_I, r_I)
+    assert np.all(d_ext <= np.array([1e-3, 1e-3, 0.05, 1e-3, 1e-3, 0.05]))
+
+
 def test_wrapped_noise_and_no_fic(oracle_mod):
     """Noise array of 1000 ms read at (ms mod 1000), and FIC off (wIE from w_IE_list)."""
     O = oracle_mod
