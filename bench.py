@@ -206,8 +206,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-sample-ms", type=int, default=60_000)
     ap.add_argument("--cpu-sims", type=int, default=16)
-    ap.add_argument("--workload", default="pop128", choices=["pop128", "surrogate"],
-                    help="pop128: the headline line; surrogate: CMA-ES with vs without the surrogate")
+    ap.add_argument("--workload", default="pop128", choices=["pop128", "surrogate", "uneven", "fcd400"],
+                    help="pop128: the headline line; surrogate: CMA-ES with vs without the surrogate; "
+                         "uneven: four populations (N 68-400) in one graph launch; fcd400: FC/FCD stress")
+    ap.add_argument("--scale", type=float, default=1.0, help="duration scale (uneven workload)")
     ap.add_argument("--generations", type=int, default=32)
     ap.add_argument("--surrogate-lambda", type=int, default=64)
     ap.add_argument("--pool", type=int, default=4)
@@ -224,6 +226,11 @@ def main():
         sys.exit(2)
     if args.workload == "surrogate":
         surrogate_experiment(args)
+        return
+    if args.workload in ("uneven", "fcd400"):
+        from bnm_cuda_amd import experiments as X
+        res = X.uneven_batch(durations_scale=args.scale) if args.workload == "uneven" else X.fcd_stress()
+        print(json.dumps(res), flush=True)
         return
 
     N, B, T = 100, 128, args.duration_ms

@@ -30,7 +30,7 @@ EXPORTS = (
     "bnm_default_config", "bnm_finalize_config", "bnm_output_ts", "bnm_n_pairs", "bnm_n_windows",
     "bnm_window_bounds", "bnm_regional_params", "bnm_analytical_fic", "bnm_make_noise", "bnm_make_noise_chunked", "bnm_run_simulations", "bnm_ctx_create",
     "bnm_ctx_destroy", "bnm_ctx_set_sc", "bnm_ctx_set_noise", "bnm_ctx_set_empirical",
-    "bnm_ctx_set_params", "bnm_ctx_run", "bnm_ctx_get_outputs", "bnm_ctx_timings",
+    "bnm_ctx_set_params", "bnm_ctx_run", "bnm_ctx_set_bold", "bnm_ctx_analyze", "bnm_ctx_get_outputs", "bnm_ctx_timings",
     "bnm_ctx_kernel_name", "bnm_ctx_tune_log", "bnm_multi_create", "bnm_multi_run",
     "bnm_multi_last_ms", "bnm_multi_destroy", "bnm_cmaes_create", "bnm_cmaes_destroy",
     "bnm_cmaes_ask", "bnm_cmaes_tell", "bnm_cmaes_best", "bnm_cmaes_sigma", "bnm_cmaes_strategy", "bnm_cmaes_iteration", "bnm_cmaes_ask_pool", "bnm_cmaes_select",
@@ -98,6 +98,8 @@ def lib():
     L.bnm_ctx_set_empirical.argtypes = [vp, pd, C.c_int, pd, C.c_int]
     L.bnm_ctx_set_params.argtypes = [vp, pd, pd, pd, pd, pd]
     L.bnm_ctx_run.argtypes = [vp]
+    L.bnm_ctx_set_bold.argtypes = [vp, pd]
+    L.bnm_ctx_analyze.argtypes = [vp]
     L.bnm_ctx_get_outputs.argtypes = [vp, pd, pd, pd] + [pd] * 7 + [pi, pi, pd, pd]
     L.bnm_ctx_timings.argtypes = [vp, pd, C.c_int]
     L.bnm_ctx_kernel_name.argtypes = [vp]
@@ -290,6 +292,14 @@ class Context:
 
     def run(self):
         _chk(lib().bnm_ctx_run(self.h))
+
+    def set_bold(self, bold):
+        bold = _f64(bold, (self.B, self.ots, self.N))
+        _chk(lib().bnm_ctx_set_bold(self.h, _p(bold, C.c_double)))
+
+    def analyze(self):
+        """FC, FCD and fit of the BOLD set by set_bold (or of the last run)."""
+        _chk(lib().bnm_ctx_analyze(self.h))
 
     def timings(self):
         out = np.zeros(6)

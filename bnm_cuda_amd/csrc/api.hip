@@ -72,6 +72,7 @@ struct bnm_ctx {
     bool fc64 = true;     // Gram precision
     bool fcbf16 = false;  // BF16 Gram operands (FP32 accumulation and storage)
     bool fcsplit = false; // ... as hi/lo pairs (three products)
+    bool have_bold = false;  // BOLD holds a run's or a caller's series
     hipStream_t st = nullptr;
     double* SC64 = nullptr;
     float* SC32 = nullptr;
@@ -703,15 +704,19 @@ static int choose_variant(bnm_ctx* x) {
 
 // Enqueues one whole run on stream `st`; with `timed` the phase events are
 // recorded (not inside graph capture).
-static int enqueue_run(bnm_ctx* x, hipStream_t st, bool timed) {
+// analyze_only: FC, FCD and fit of the BOLD already in the context (no FIC
+// initialisation, no integration, no FIC penalty)
+static int enqueue_run(bnm_ctx* x, hipStream_t st, bool timed, bool analyze_only = false) {
     const bnm_config& c = x->cfg;
     if (timed) HIPCHK(hipEventRecord(x->ev[0], st));
-    if (c.do_fic && c.analytical_fic)
+    if (c.do_fic && c.analytical_fic && !analyze_only)
         HIPCHK(launch_fic_analytical(x->SC64, x->N, x->B, x->G, x->wEE, x->wEI, x->wIEfic,
                                      x->unstable, st));
     if (timed) HIPCHK(hipEventRecord(x->ev[1], st));
-    const SimArgs a = make_args(x);
-    HIPCHK(launch_variant(x, x->variant, a, st));
+    if (!analyze_only) {
+        const SimArgs a = make_args(x);
+        HIPCHK(launch_variant(x, x->variant, a, st));
+    }
     if (timed) HIPCHK(hipEventRecord(x->ev[2], st));
     if (!c.sim_only) {
         if (x->f64) {
@@ -765,7 +770,7 @@ static int enqueue_run(bnm_ctx* x, hipStream_t st, bool timed) {
     g.sim_only = c.sim_only;
     g.no_fcd = c.no_fcd;
     g.use_fc_diff = c.use_fc_diff;
-    g.calc_penalty = c.calculate_fic_penalty;
+    g.calc_penalty = c.calculate_fic_penalty && !analyze_only;
     g.fic_reject = c.fic_reject_failed;
     g.max_cost = c.max_cost;
     g.fic_scale = c.fic_penalty_scale;
@@ -805,6 +810,38 @@ static bool ready(const bnm_ctx* x) {
     return x->have_sc && x->have_noise && x->have_params && (x->cfg.sim_only || x->have_emp);
 }
 
+int bnm_ctx_set_bold(bnm_ctx* x, const double* bold) {
+    if (!x || !bold) return fail(BNM_E_ARG, "null argument");
+    const size_t n = (size_t)x->B * x->ots * x->N;
+    if (x->f64) {
+        HIPCHK(hipMemcpy(x->bold, bold, n * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> f(bold, bold + n);
+        HIPCHK(hipMemcpy(x->bold, f.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    }
+    x->have_bold = true;
+    return BNM_OK;
+}
+
+int bnm_ctx_analyze(bnm_ctx* x) {
+    if (!x) return fail(BNM_E_ARG, "null context");
+    if (x->cfg.sim_only) return fail(BNM_E_ARG, "sim_only context has no FC/FCD to compute");
+    if (!x->have_emp || !x->have_bold) return fail(BNM_E_ARG, "empirical target and BOLD must be set first");
+    int r = enqueue_run(x, x->st, true, true);
+    if (r) return r;
+    HIPCHK(hipStreamSynchronize(x->st));
+    float ms = 0;
+    for (int i = 0; i < 4; ++i) {
+        HIPCHK(hipEventElapsedTime(&ms, x->ev[i], x->ev[i + 1]));
+        x->timings[i] = ms;
+    }
+    HIPCHK(hipEventElapsedTime(&ms, x->ev[4], x->ev[5]));
+    x->timings[4] = ms;
+    HIPCHK(hipEventElapsedTime(&ms, x->ev[0], x->ev[5]));
+    x->timings[5] = ms;
+    return BNM_OK;
+}
+
 int bnm_ctx_run(bnm_ctx* x) {
     if (!x) return fail(BNM_E_ARG, "null context");
     if (!ready(x))
@@ -831,6 +868,7 @@ int bnm_ctx_run(bnm_ctx* x) {
         HIPCHK(hipEventElapsedTime(&ms, x->ev[0], x->ev[5]));
         for (int i = 0; i < 5; ++i) x->timings[i] = -1;
         x->timings[5] = ms;
+        x->have_bold = true;
         return BNM_OK;
     }
     r = enqueue_run(x, x->st, true);
@@ -845,6 +883,7 @@ int bnm_ctx_run(bnm_ctx* x) {
     x->timings[4] = ms;
     HIPCHK(hipEventElapsedTime(&ms, x->ev[0], x->ev[5]));
     x->timings[5] = ms;
+    x->have_bold = true;
     return BNM_OK;
 }
 

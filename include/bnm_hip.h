@@ -190,6 +190,14 @@ int bnm_ctx_get_outputs(bnm_ctx* ctx, double* corr, double* fic_penalties, doubl
                         double* BOLD, double* S_E, double* I_E, double* r_E, double* S_I,
                         double* I_I, double* r_I, int* FIC_failed, int* fic_n_trials,
                         double* fc_trils, double* fcd_trils);
+/* FC, FCD and fit of a given BOLD array (no integration): bnm_ctx_set_bold
+ * copies BOLD [B*output_ts*N] (time-major per simulation, the layout
+ * bnm_ctx_get_outputs returns) into the context, bnm_ctx_analyze computes
+ * the moments, FC, window FC, FCD, KS and cost (no FIC penalty) with the
+ * context's fc_precision.  Needs the empirical target.  Also the way to turn
+ * empirical BOLD into emp_FC / emp_FCD.  No counterpart in the reference. */
+int bnm_ctx_set_bold(bnm_ctx* ctx, const double* BOLD);
+int bnm_ctx_analyze(bnm_ctx* ctx);
 /* Device time (ms) of the last bnm_ctx_run, per phase:
  * [0] analytical FIC, [1] integration, [2] BOLD moments + FC + window FC,
  * [3] FCD, [4] fit (KS, Pearson, penalty), [5] whole run.

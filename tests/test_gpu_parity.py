@@ -202,6 +202,25 @@ def test_f32_ei_split_kernel_within_bounds(oracle_mod, case, ref, ls, monkeypatc
     assert np.all(d_ext <= np.array([1e-3, 1e-3, 0.05, 1e-3, 1e-3, 0.05]))
 
 
+@pytest.mark.parametrize("fcp", [bnm.FC_F64, bnm.FC_F32])
+def test_analyze_given_bold_matches_oracle(case, ref, fcp):
+    """bnm_ctx_set_bold + bnm_ctx_analyze on the oracle's BOLD reproduce the
+    oracle's FC, FCD and cost (no integration, no SC / noise / parameters)."""
+    cfg = _cfg(case, bnm.PREC_F64, bnm.KERNEL_FAST, fc_precision=fcp)
+    with bnm.Context(cfg) as c:
+        c.set_empirical(case["emp_fc"], case["emp_fcd"])
+        c.set_bold(ref["bold"])
+        c.analyze()
+        g = c.outputs()
+        t = c.timings()
+    tol = 1e-9 if fcp == bnm.FC_F64 else 1e-4
+    print("analyze timings", t)
+    np.testing.assert_allclose(g["fc"], ref["fc"], atol=tol)
+    np.testing.assert_allclose(g["fcd"], ref["fcd"], atol=tol * 10)
+    np.testing.assert_allclose(g["corr"], ref["corr"], atol=tol * 10)
+    np.testing.assert_array_equal(g["bold"], ref["bold"])
+
+
 def test_wrapped_noise_and_no_fic(oracle_mod):
     """Noise array of 1000 ms read at (ms mod 1000), and FIC off (wIE from w_IE_list)."""
     O = oracle_mod
