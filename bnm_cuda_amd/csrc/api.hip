@@ -60,7 +60,8 @@ int dalloc(T** p, size_t n) {
 // One launchable integration kernel shape.
 struct Variant {
     int kind;   // 0 split (SC in registers), 1 stream (SC from L2), 2 mfma, 3 plain,
-                // 4 split with the E/I pools on separate lanes (FP32)
+                // 4 split with the E/I pools on separate lanes (FP32),
+                // 5 stream with register/LDS-resident SC (FP32)
     int ls, ks, mode;
 };
 
@@ -78,6 +79,8 @@ struct bnm_ctx {
     float* SC32 = nullptr;
     double* SCt64 = nullptr;  // transposed copies for the streamed large-N kernel
     float* SCt32 = nullptr;
+    float* SCc32 = nullptr;   // chunked [k/16][j][k%16] copy for the hybrid kernel
+    size_t lds_max = 64 * 1024;  // dynamic LDS a workgroup may use
     float* noise = nullptr;
     long long noise_cap = 0;
     double *G = nullptr, *wEE = nullptr, *wEI = nullptr, *wIE = nullptr, *wIEfic = nullptr,
@@ -102,7 +105,7 @@ struct bnm_ctx {
     Variant variant{0, 4, 1, 0};
     hipGraphExec_t graph_exec = nullptr;
     bool have_sc = false, have_noise = false, have_emp = false, have_params = false;
-    int fast_ls = 0, fast_ks = 0, stream_ks = 0, force_stream = 0, no_ei = 0, ei_ls = 0;
+    int fast_ls = 0, fast_ks = 0, stream_ks = 0, force_stream = 0, no_ei = 0, ei_ls = 0, no_hyb = 0, hyb_ks = 0;
 };
 
 extern "C" {
@@ -253,7 +256,7 @@ int bnm_make_noise_chunked(unsigned seed, long long count, float* out, int threa
 
 void bnm_ctx_destroy(bnm_ctx* x) {
     if (!x) return;
-    void* bufs[] = {x->SC64, x->SC32, x->SCt64, x->SCt32, x->noise, x->G, x->wEE, x->wEI, x->wIE, x->wIEfic,
+    void* bufs[] = {x->SC64, x->SC32, x->SCt64, x->SCt32, x->SCc32, x->noise, x->G, x->wEE, x->wEI, x->wIE, x->wIEfic,
                     x->wIEscratch, x->bold, x->ext, x->failed, x->trials, x->unstable,
                     x->seg_start, x->seg_len, x->pidx, x->mean, x->inv, x->wmean, x->winv, x->fc,
                     x->wfc, x->fcd, x->emp_c, x->ks_u, x->ks_fe, x->bins, x->nanf, x->corr, x->pen};
@@ -274,7 +277,12 @@ static int ctx_alloc(bnm_ctx* x) {
     if ((r = dalloc(&x->SC64, N * N))) return r;
     if ((r = dalloc(&x->SC32, N * N))) return r;
     if ((r = dalloc(&x->SCt64, N * N))) return r;
-    if ((r = dalloc(&x->SCt32, N * N))) return r;
+    // kSCtPad zero rows after the N rows: the hybrid kernel's padded k-ranges
+    // read them without bounds checks
+    if ((r = dalloc(&x->SCt32, (N + kSCtPad) * N))) return r;
+    HIPCHK(hipMemset(x->SCt32, 0, (N + kSCtPad) * N * sizeof(float)));
+    if ((r = dalloc(&x->SCc32, (N + kSCtPad) * N))) return r;
+    HIPCHK(hipMemset(x->SCc32, 0, (N + kSCtPad) * N * sizeof(float)));
     if ((r = dalloc(&x->G, B))) return r;
     if ((r = dalloc(&x->wEE, B * N))) return r;
     if ((r = dalloc(&x->wEI, B * N))) return r;
@@ -420,6 +428,14 @@ bnm_ctx* bnm_ctx_create(const bnm_config* cfg, int* status) {
     x->force_stream = env_int("BNM_FORCE_STREAM", 0);
     x->no_ei = env_int("BNM_NO_EI", 0);
     x->ei_ls = env_int("BNM_EI_LS", 0);
+    x->no_hyb = env_int("BNM_NO_HYB", 0);
+    x->hyb_ks = env_int("BNM_HYB_KS", 0);
+    {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+            x->lds_max = (size_t)v;
+    }
     x->autotune = c.autotune;
     x->tune_ms = env_int("BNM_TUNE_MS", 300);
     if (hipStreamCreateWithFlags(&x->st, hipStreamNonBlocking) != hipSuccess) {
@@ -452,6 +468,11 @@ int bnm_ctx_set_sc(bnm_ctx* x, const double* SC) {
     HIPCHK(hipMemcpy(x->SC32, f.data(), n * sizeof(float), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(x->SCt64, dt.data(), n * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(x->SCt32, ft.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    // chunked copy: SCc[((k/16)*N + j)*16 + k%16] = SC[j][k]; k >= N stays zero
+    std::vector<float> fc(((N + 15) / 16) * 16 * N, 0.0f);
+    for (size_t j = 0; j < N; ++j)
+        for (size_t k = 0; k < N; ++k) fc[((k / 16) * N + j) * 16 + k % 16] = (float)SC[j * N + k];
+    HIPCHK(hipMemcpy(x->SCc32, fc.data(), fc.size() * sizeof(float), hipMemcpyHostToDevice));
     x->have_sc = true;
     return BNM_OK;
 }
@@ -540,6 +561,7 @@ static bool variant_ok(const bnm_ctx* x, const Variant& v) {
         case 1: return stream_shape(x->N, v.ks, el).ok;
         case 2: return !x->f64 && mfma_shape(x->N, v.mode).ok;
         case 4: return !x->f64 && ei_shape(x->N, v.ls).ok;
+        case 5: return !x->f64 && hyb_shape(x->N, v.ks, x->lds_max).ok;
         default: return true;
     }
 }
@@ -561,6 +583,10 @@ static std::string variant_name(const bnm_ctx* x, const Variant& v) {
         const MfmaShape s = mfma_shape(x->N, v.mode);
         std::snprintf(name, sizeof name, "mfma_%s(sims/block=16,waves=%d,KQ=%d,threads=%d)",
                       mn[v.mode], s.waves, s.KQ, s.threads);
+    } else if (v.kind == 5) {
+        const FastShape g = hyb_shape(x->N, v.ks, x->lds_max);
+        std::snprintf(name, sizeof name, "hyb_f32(KS=%d,NW=%d,kc=%d,regs=%d,lds=%d,streamed=%d,threads=%d)", g.KS,
+                      g.NW, g.kc, g.KC, g.SS, g.kc - g.KC - g.SS, g.threads);
     } else if (v.kind == 4) {
         const FastShape s = ei_shape(x->N, v.ls);
         std::snprintf(name, sizeof name, "split_ei_f32(LS=%d,NW=%d,KC=%d,SS=%d,threads=%d)", s.LS, s.NW, s.KC,
@@ -581,6 +607,7 @@ static hipError_t launch_variant(bnm_ctx* x, const Variant& v, const SimArgs& a,
                           : launch_sim_stream<float>(x->SCt32, a, v.ks, st);
         case 2: return launch_sim_mfma(x->SC32, a, v.mode, st);
         case 4: return launch_sim_ei(x->SC32, a, v.ls, st);
+        case 5: return launch_sim_hyb(x->SCc32, a, v.ks, x->lds_max, st);
         default:
             return x->f64 ? launch_sim_plain<double>(x->SC64, a, st)
                           : launch_sim_plain<float>(x->SC32, a, st);
@@ -605,6 +632,10 @@ static std::vector<Variant> candidates(const bnm_ctx* x) {
         add({2, 0, 0, 0});
         return out;
     }
+    if (x->hyb_ks > 0 && !x->f64) {
+        add({5, 1, x->hyb_ks, 0});
+        if (!out.empty()) return out;
+    }
     if (x->ei_ls > 0 && !x->f64) {
         add({4, x->ei_ls, 1, 0});
         if (!out.empty()) return out;
@@ -626,8 +657,11 @@ static std::vector<Variant> candidates(const bnm_ctx* x) {
     if (out.empty() || x->force_stream) {
         if (x->stream_ks > 0) {
             add({1, 1, x->stream_ks, 0});
+            if (!x->f64 && !x->no_hyb) add({5, 1, x->stream_ks, 0});
         } else {
             for (int k : {16, 8, 4, 2, 1}) add({1, 1, k, 0});
+            if (!x->f64 && !x->no_hyb)
+                for (int k : {8, 4, 2, 1}) add({5, 1, k, 0});
         }
     }
     // the MFMA kernel implements FP32 too; the tuner may pick it for large populations

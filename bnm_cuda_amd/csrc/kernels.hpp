@@ -27,6 +27,14 @@ template <class R>
 hipError_t launch_sim_fast(const R* SC, const SimArgs& a, int LS, int KS, hipStream_t st);
 // large N: SC streamed from a transposed copy SCt[k][j] in global memory
 FastShape stream_shape(int N, int KS, size_t elem);
+// zero k-rows the chunked FP32 SC carries after its N rows (>= the k-range
+// padding of any hybrid shape: KS * 16 with KS <= 16)
+constexpr size_t kSCtPad = 256;
+// FP32 streaming kernel with register- and LDS-resident SC (N > 128); the
+// shape's KC field is the register part, SS the LDS part (values per lane)
+FastShape hyb_shape(int N, int KS, size_t lds_bytes);
+// SCc: chunked layout SCc[((k/16)*N + j)*16 + k%16] = SC[j][k], zero for k >= N
+hipError_t launch_sim_hyb(const float* SCc, const SimArgs& a, int KS, size_t lds_bytes, hipStream_t st);
 // FP32 split kernel with the node update divided between lanes by population
 FastShape ei_shape(int N, int LS);
 hipError_t launch_sim_ei(const float* SC, const SimArgs& a, int LS, hipStream_t st);

@@ -10,6 +10,8 @@
 //                  of a node split over LS lanes of a wave and KS wave groups
 //                  (see the kernel's comment); one barrier per substep when
 //                  KS == 1.
+#include <algorithm>
+
 #include "kernels.hpp"
 #include "model.hpp"
 
@@ -610,6 +612,167 @@ __global__ __launch_bounds__(1024) void k_sim_split_g(const R* __restrict__ SCt,
 }
 
 // ---------------------------------------------------------------------------
+// hyb (FP32, N > 128): the streaming kernel with most of SC resident.  Lane
+// (j, ks) owns SC[j][k0 .. k0+kc); the first RR values stay in VGPRs, the
+// next LL in LDS (float4 per lane per 4 k, laid out [kk/4][thread] so a
+// wave reads 1 KB contiguous), and only the rest is streamed from L2 each
+// substep, 16 k per chunk, double-buffered, the first chunk issued before
+// the resident FMAs.  SC comes in a chunked layout SCc[k/16][j][k%16], so a
+// lane's 16 values of a chunk are 64 contiguous bytes (four 16-byte buffer
+// loads; a wave reads 4 KB contiguous), addressed by one VGPR column offset
+// and one SGPR chunk offset.  Per-CU L1 bandwidth (64 B/clk) is what bounds
+// the plain streaming kernel at N = 400 (640 KB of SC per substep);
+// residency cuts the streamed part to what LDS and registers cannot hold.
+// ---------------------------------------------------------------------------
+template <int RR, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_sim_hyb(const float* __restrict__ SCc, SimArgs a, int NW, int KS,
+                                                  int kc, int LL) {
+    constexpr int CH = 16;
+    static_assert(RR % CH == 0, "register part must be whole chunks");
+    const int sim = blockIdx.x;
+    const int N = a.N;
+    const int T = blockDim.x;
+    const int tid = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    const int nw = wave % NW, ks = wave / NW;
+    const int j = nw * 64 + lane;
+    const bool comp = ks == 0 && j < N;
+    const int k0 = ks * kc;
+    const int ldS = KS * kc;
+    const int jc = j < N ? j : N - 1;
+    extern __shared__ unsigned char smem_raw[];
+    float* Sb = reinterpret_cast<float*>(smem_raw);   // [2][ldS]
+    float* part = Sb + 2 * ldS;                        // [KS][NW*64]
+    float4* SCl = reinterpret_cast<float4*>(part + KS * NW * 64);   // [LL/4][T]
+    for (int i = tid; i < 2 * ldS; i += T) Sb[i] = (i % ldS) < N ? 0.001f : 0.0f;
+    // chunks k/16 >= N/16 hold zeros (padding); the resource's size makes any
+    // read past the array return 0
+    const __amdgpu_buffer_rsrc_t scr = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float*>(SCc), (short)0, (int)((N + (int)kSCtPad) * N * sizeof(float)), 0x00020000);
+    const int voff = jc * CH * (int)sizeof(float);
+    auto chunk = [&](int kb, float* w) {     // the lane's 16 values of chunk kb
+        const int so = kb * N * CH * (int)sizeof(float);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const auto v = __builtin_amdgcn_raw_buffer_load_b128(scr, voff + 16 * q, so, 0);
+            w[4 * q + 0] = __uint_as_float(v[0]);
+            w[4 * q + 1] = __uint_as_float(v[1]);
+            w[4 * q + 2] = __uint_as_float(v[2]);
+            w[4 * q + 3] = __uint_as_float(v[3]);
+        }
+    };
+    float wr[RR];
+#pragma unroll
+    for (int c = 0; c < RR / CH; ++c) chunk(k0 / CH + c, wr + c * CH);
+    for (int c = 0; c < LL / CH; ++c) {
+        float w[CH];
+        chunk((k0 + RR) / CH + c, w);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            SCl[(size_t)(c * 4 + q) * T + tid] = float4{w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]};
+    }
+    const int gb = (k0 + RR + LL) / CH;   // first streamed chunk
+    const int nch = (kc - RR - LL) / CH;  // streamed chunks
+
+    Lane<float> L;
+    if (comp) load_params(a, sim, j, L);
+    const float gp = gain<float>(a.G[sim]);
+    int ts = 0, vol = 0, trial = 0, buf = 0;
+    bool adjusting = a.adjust_fic != 0, failed = false;
+    const unsigned dur = (unsigned)(a.i_end - a.i_start + 1);
+    const float2* noise2 = reinterpret_cast<const float2*>(a.noise);
+    __syncthreads();
+    while (ts <= a.time_steps) {
+        const long long ms = ts % a.noise_period;
+        const float2* nzp = noise2 + (size_t)ms * 10 * N + j;
+#pragma unroll 1
+        for (int s = 0; s < 10; ++s) {
+            const float2 nz = comp ? nzp[(size_t)s * N] : float2{0.0f, 0.0f};
+            float w0[CH], w1[CH];
+            if (nch > 0) chunk(gb, w0);
+            const float4* Sr = reinterpret_cast<const float4*>(Sb + buf * ldS + k0);
+            float acc[4] = {0, 0, 0, 0};
+#pragma unroll
+            for (int q4 = 0; q4 < RR / 4; ++q4) {
+                const float4 v = Sr[q4];
+                acc[0] = fmaf(wr[4 * q4 + 0], v.x, acc[0]);
+                acc[1] = fmaf(wr[4 * q4 + 1], v.y, acc[1]);
+                acc[2] = fmaf(wr[4 * q4 + 2], v.z, acc[2]);
+                acc[3] = fmaf(wr[4 * q4 + 3], v.w, acc[3]);
+            }
+            for (int q4 = 0; q4 < LL / 4; ++q4) {
+                const float4 v = Sr[RR / 4 + q4];
+                const float4 w = SCl[(size_t)q4 * T + tid];
+                acc[0] = fmaf(w.x, v.x, acc[0]);
+                acc[1] = fmaf(w.y, v.y, acc[1]);
+                acc[2] = fmaf(w.z, v.z, acc[2]);
+                acc[3] = fmaf(w.w, v.w, acc[3]);
+            }
+            const float4* Sg = Sr + (RR + LL) / 4;
+            for (int c = 0; c < nch; c += 2) {
+                if (c + 1 < nch) chunk(gb + c + 1, w1);
+#pragma unroll
+                for (int q4 = 0; q4 < CH / 4; ++q4) {
+                    const float4 v = Sg[c * (CH / 4) + q4];
+                    acc[0] = fmaf(w0[4 * q4 + 0], v.x, acc[0]);
+                    acc[1] = fmaf(w0[4 * q4 + 1], v.y, acc[1]);
+                    acc[2] = fmaf(w0[4 * q4 + 2], v.z, acc[2]);
+                    acc[3] = fmaf(w0[4 * q4 + 3], v.w, acc[3]);
+                }
+                if (c + 2 < nch) chunk(gb + c + 2, w0);
+                if (c + 1 < nch) {
+#pragma unroll
+                    for (int q4 = 0; q4 < CH / 4; ++q4) {
+                        const float4 v = Sg[(c + 1) * (CH / 4) + q4];
+                        acc[0] = fmaf(w1[4 * q4 + 0], v.x, acc[0]);
+                        acc[1] = fmaf(w1[4 * q4 + 1], v.y, acc[1]);
+                        acc[2] = fmaf(w1[4 * q4 + 2], v.z, acc[2]);
+                        acc[3] = fmaf(w1[4 * q4 + 3], v.w, acc[3]);
+                    }
+                }
+            }
+            float c = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+            if (KS > 1) {
+                if (ks > 0) part[ks * NW * 64 + j] = c;
+                __syncthreads();
+                if (ks == 0)
+                    for (int t = 1; t < KS; ++t) c += part[t * NW * 64 + j];
+            }
+            if (comp) {
+                Node<float>::step(c, gp, L.wEE, L.wEI, L.wIE, nz.x, nz.y, L.SE, L.SI, L.IE, L.II, L.rE, L.rI);
+                Sb[(buf ^ 1) * ldS + j] = L.SE;
+            }
+            __syncthreads();
+            buf ^= 1;
+        }
+        if (comp) L.bw.step(L.SE);
+        if (ts % a.bold_tr == 0) {
+            if (comp) sample(a, sim, j, vol, L);
+            ++vol;
+        }
+        ++ts;
+        if (adjusting) {
+            if (comp && ts >= a.i_start && ts <= a.i_end) L.mIE += (double)L.IE;
+            if (ts == a.i_end) {
+                const bool out = comp && fic_decide<float>(L.mIE, dur, trial, a.max_fic_trials, L.wIE, L.delta);
+                if (__syncthreads_or(out)) {
+                    if (trial < a.max_fic_trials) {
+                        if (comp) { L.reset(); Sb[buf * ldS + j] = 0.001f; }
+                        ts = 0; vol = 0; ++trial;
+                        __syncthreads();
+                    } else {
+                        failed = true; adjusting = false;
+                    }
+                } else {
+                    adjusting = false;
+                }
+            }
+        }
+    }
+    finish(a, sim, j, comp, vol, trial, failed, L);
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 template <class R>
@@ -747,6 +910,53 @@ hipError_t launch_sim_stream(const R* SCt, const SimArgs& a, int KS, hipStream_t
     const FastShape s = stream_shape(a.N, KS, sizeof(R));
     if (!s.ok) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_sim_split_g<R>, dim3(a.B), dim3(s.threads), s.smem, st, SCt, a, s.NW, KS, s.kc);
+    return hipGetLastError();
+}
+
+// Hybrid shape (at most 512 threads): RR register-resident values per lane
+// (96 with two waves per SIMD, 160 with one), then as many LDS
+// values as `lds_bytes` holds, rounded so the streamed rest is whole chunks.
+FastShape hyb_shape(int N, int KS, size_t lds_bytes) {
+    FastShape s = stream_shape(N, KS, sizeof(float));
+    // RR: the register part the workgroup size leaves room for (one or two
+    // waves per SIMD), capped by the lane's k-range
+    s.KC = s.threads <= 256 ? 160 : 96;
+    while (s.KC > s.kc) s.KC -= 16;
+    const size_t base = (2 * (size_t)KS * s.kc + (size_t)KS * s.NW * 64) * sizeof(float);
+    int LL = 0;
+    if (lds_bytes > base + 16) LL = (int)((lds_bytes - base) / ((size_t)s.threads * sizeof(float)));
+    LL = std::min(LL, s.kc - s.KC) / 16 * 16;   // whole chunks
+    s.SS = LL;
+    s.smem = base + (size_t)LL * s.threads * sizeof(float);
+    s.ok = s.threads <= 512 && s.smem <= lds_bytes && (s.kc - s.KC - LL) % 16 == 0 && s.KC % 16 == 0 &&
+           (s.KC == 160 || s.KC == 96 || s.KC == 64 || s.KC == 32 || s.KC == 16);
+    return s;
+}
+
+template <int RR>
+static void launch_hyb(const float* SCc, const SimArgs& a, const FastShape& s, hipStream_t st) {
+    constexpr int MAXT = RR <= 96 ? 512 : 256;
+    static bool attr = false;   // allow more than 64 KB of dynamic LDS
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void*)k_sim_hyb<RR, MAXT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
+        attr = true;
+    }
+    hipLaunchKernelGGL((k_sim_hyb<RR, MAXT>), dim3(a.B), dim3(s.threads), s.smem, st, SCc, a, s.NW, s.KS,
+                       s.kc, s.SS);
+}
+
+hipError_t launch_sim_hyb(const float* SCc, const SimArgs& a, int KS, size_t lds_bytes, hipStream_t st) {
+    const FastShape s = hyb_shape(a.N, KS, lds_bytes);
+    if (!s.ok) return hipErrorInvalidValue;
+    switch (s.KC) {
+        case 16: launch_hyb<16>(SCc, a, s, st); break;
+        case 32: launch_hyb<32>(SCc, a, s, st); break;
+        case 64: launch_hyb<64>(SCc, a, s, st); break;
+        case 96: launch_hyb<96>(SCc, a, s, st); break;
+        case 160: launch_hyb<160>(SCc, a, s, st); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

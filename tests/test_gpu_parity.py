@@ -335,9 +335,50 @@ def test_n400_streamed_f32_vs_f64():
             c.set_params(G, wEE, wEI)
             c.run()
             print(c.kernel_name(), c.timings())
-            assert c.kernel_name().startswith("stream")
+            assert c.kernel_name().startswith("stream" if prec == bnm.PREC_F64 else ("stream", "hyb"))
             out[prec] = c.outputs()
     a, b = out[bnm.PREC_F32], out[bnm.PREC_F64]
     np.testing.assert_array_equal(a["fic_trials"], b["fic_trials"])
     assert np.abs(a["ext"][:, 2] - b["ext"][:, 2]).max() <= 0.05
     assert np.abs(a["fc"] - b["fc"]).max() <= 1e-3
+
+
+@pytest.mark.parametrize("N_big,ks", [(200, 1), (200, 2), (400, 1)])
+def test_hybrid_resident_sc_kernel_vs_f64(monkeypatch, N_big, ks):
+    """FP32 kernel with register- and LDS-resident SC and the rest streamed
+    (chunked SC layout, buffer loads) against the FP64 streamed kernel on
+    the same inputs: same FIC trials, summary statistics within bounds,
+    early BOLD close (a wrong chunk offset or LDS slot would not be)."""
+    B4, T4 = 4, 30_000
+    SC = W.synthetic_sc(N_big, 0.3, 17)
+    rng = np.random.default_rng(5)
+    G = np.array([0.8, 1.5, 2.5, 0.5])
+    wEE = W.heterogeneous(np.array([0.2, 0.25, 0.15, 0.3]), rng.normal(0, 1, N_big), 0.02)
+    wEI = W.heterogeneous(np.array([0.15, 0.2, 0.25, 0.1]), rng.normal(0, 1, N_big), 0.02)
+    noise = bnm.make_noise_chunked(3, (T4 + 1) * 20 * N_big)
+    emp_fc = rng.uniform(-0.2, 0.8, bnm.n_pairs(N_big, False))
+    emp_fcd = rng.uniform(-0.5, 1, 40)
+    out = {}
+    for prec in (bnm.PREC_F64, bnm.PREC_F32):
+        if prec == bnm.PREC_F32:
+            monkeypatch.setenv("BNM_HYB_KS", str(ks))
+        cfg = bnm.default_config(nodes=N_big, n_sims=B4, time_steps=T4, bold_tr=1000, n_vols_remove=0,
+                                 window_size=10, window_step=2, exc_interhemispheric=0,
+                                 noise_period_ms=T4 + 1, precision=prec, kernel=bnm.KERNEL_FAST,
+                                 fc_precision=bnm.FC_F64)
+        with bnm.Context(cfg) as c:
+            c.set_sc(SC); c.set_noise(noise); c.set_empirical(emp_fc, emp_fcd)
+            c.set_params(G, wEE, wEI)
+            c.run()
+            print(c.kernel_name(), c.timings())
+            if prec == bnm.PREC_F32:
+                assert c.kernel_name().startswith("hyb_f32(KS=%d" % ks)
+            out[prec] = c.outputs()
+    a, b = out[bnm.PREC_F32], out[bnm.PREC_F64]
+    rel = _bold_rel(a["bold"][:, :11], b["bold"][:, :11])
+    d_re = np.abs(a["ext"][:, 2] - b["ext"][:, 2]).max()
+    d_fc = np.abs(a["fc"] - b["fc"]).max()
+    print(f"N={N_big} KS={ks}: trials {a['fic_trials']} vs {b['fic_trials']}, BOLD rel {rel:.3g}, "
+          f"d rE {d_re:.3g}, d FC {d_fc:.3g}")
+    np.testing.assert_array_equal(a["fic_trials"], b["fic_trials"])
+    assert rel <= 1e-4 and d_re <= 0.05 and d_fc <= 1e-3
