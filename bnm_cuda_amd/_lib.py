@@ -30,7 +30,7 @@ EXPORTS = (
     "bnm_default_config", "bnm_finalize_config", "bnm_output_ts", "bnm_n_pairs", "bnm_n_windows",
     "bnm_window_bounds", "bnm_regional_params", "bnm_analytical_fic", "bnm_make_noise", "bnm_make_noise_chunked", "bnm_run_simulations", "bnm_ctx_create",
     "bnm_ctx_destroy", "bnm_ctx_set_sc", "bnm_ctx_set_noise", "bnm_ctx_set_empirical",
-    "bnm_ctx_set_params", "bnm_ctx_run", "bnm_ctx_set_bold", "bnm_ctx_analyze", "bnm_ctx_get_outputs", "bnm_ctx_timings",
+    "bnm_ctx_set_params", "bnm_ctx_run", "bnm_ctx_set_bold", "bnm_ctx_analyze", "bnm_ctx_get_hemi_fcd", "bnm_ctx_get_outputs", "bnm_ctx_timings",
     "bnm_ctx_kernel_name", "bnm_ctx_tune_log", "bnm_multi_create", "bnm_multi_run",
     "bnm_multi_last_ms", "bnm_multi_destroy", "bnm_cmaes_create", "bnm_cmaes_destroy",
     "bnm_cmaes_ask", "bnm_cmaes_tell", "bnm_cmaes_best", "bnm_cmaes_sigma", "bnm_cmaes_strategy", "bnm_cmaes_iteration", "bnm_cmaes_ask_pool", "bnm_cmaes_select",
@@ -58,7 +58,7 @@ class Config(C.Structure):
         ("use_fc_diff", C.c_int), ("max_cost", C.c_double), ("fic_penalty_scale", C.c_double),
         ("fic_reject_failed", C.c_int), ("analytical_fic", C.c_int), ("precision", C.c_int),
         ("fc_precision", C.c_int), ("kernel", C.c_int), ("sims_per_block", C.c_int),
-        ("autotune", C.c_int), ("use_graph", C.c_int),
+        ("autotune", C.c_int), ("use_graph", C.c_int), ("save_hemis", C.c_int),
     ]
 
 
@@ -100,6 +100,7 @@ def lib():
     L.bnm_ctx_run.argtypes = [vp]
     L.bnm_ctx_set_bold.argtypes = [vp, pd]
     L.bnm_ctx_analyze.argtypes = [vp]
+    L.bnm_ctx_get_hemi_fcd.argtypes = [vp, pd, pd]
     L.bnm_ctx_get_outputs.argtypes = [vp, pd, pd, pd] + [pd] * 7 + [pi, pi, pd, pd]
     L.bnm_ctx_timings.argtypes = [vp, pd, C.c_int]
     L.bnm_ctx_kernel_name.argtypes = [vp]
@@ -296,6 +297,13 @@ class Context:
     def set_bold(self, bold):
         bold = _f64(bold, (self.B, self.ots, self.N))
         _chk(lib().bnm_ctx_set_bold(self.h, _p(bold, C.c_double)))
+
+    def hemi_fcd(self):
+        """(left FCD, right FCD) [B, n_window_pairs] of a save_hemis context."""
+        l = np.zeros((self.B, self.nwp))
+        r = np.zeros((self.B, self.nwp))
+        _chk(lib().bnm_ctx_get_hemi_fcd(self.h, _p(l, C.c_double), _p(r, C.c_double)))
+        return l, r
 
     def analyze(self):
         """FC, FCD and fit of the BOLD set by set_bold (or of the last run)."""

@@ -91,6 +91,10 @@ struct bnm_ctx {
     int *seg_start = nullptr, *seg_len = nullptr, *pidx = nullptr;
     double *mean = nullptr, *inv = nullptr, *wmean = nullptr, *winv = nullptr;
     void *fc = nullptr, *wfc = nullptr, *fcd = nullptr;
+    // hemisphere FCDs (save_hemis): first / last n_hemi edges of each window FC
+    void *fcd_h[2] = {nullptr, nullptr};
+    double *wmean_h[2] = {nullptr, nullptr}, *winv_h[2] = {nullptr, nullptr};
+    int n_hemi = 0;
     double *emp_c = nullptr, *ks_u = nullptr, *ks_fe = nullptr;
     double emp_mean = 0, emp_ssd = 0;
     int* bins = nullptr;
@@ -259,7 +263,8 @@ void bnm_ctx_destroy(bnm_ctx* x) {
     void* bufs[] = {x->SC64, x->SC32, x->SCt64, x->SCt32, x->SCc32, x->noise, x->G, x->wEE, x->wEI, x->wIE, x->wIEfic,
                     x->wIEscratch, x->bold, x->ext, x->failed, x->trials, x->unstable,
                     x->seg_start, x->seg_len, x->pidx, x->mean, x->inv, x->wmean, x->winv, x->fc,
-                    x->wfc, x->fcd, x->emp_c, x->ks_u, x->ks_fe, x->bins, x->nanf, x->corr, x->pen};
+                    x->wfc, x->fcd, x->emp_c, x->ks_u, x->ks_fe, x->bins, x->nanf, x->corr, x->pen,
+                    x->fcd_h[0], x->fcd_h[1], x->wmean_h[0], x->wmean_h[1], x->winv_h[0], x->winv_h[1]};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     for (auto& e : x->ev)
@@ -330,6 +335,14 @@ static int ctx_alloc(bnm_ctx* x) {
             if ((r = dalloc(&x->wmean, B * x->nw))) return r;
             if ((r = dalloc(&x->winv, B * x->nw))) return r;
             HIPCHK(hipMalloc(&x->fcd, B * (size_t)x->nwp * fsz));
+            if (c.save_hemis) {
+                x->n_hemi = (x->N / 2) * (x->N / 2 - 1) / 2;
+                for (int h = 0; h < 2; ++h) {
+                    HIPCHK(hipMalloc(&x->fcd_h[h], B * (size_t)x->nwp * fsz));
+                    if ((r = dalloc(&x->wmean_h[h], B * x->nw))) return r;
+                    if ((r = dalloc(&x->winv_h[h], B * x->nw))) return r;
+                }
+            }
             if ((r = dalloc(&x->nanf, B))) return r;
         }
     }
@@ -826,6 +839,30 @@ static int enqueue_run(bnm_ctx* x, hipStream_t st, bool timed, bool analyze_only
                 HIPCHK((launch_gram_fcd<float, float, float>((const float*)x->wfc, x->B, x->nw, x->np,
                                                              x->wmean, x->winv, (float*)x->fcd, st)));
         }
+        // left / right FCD (bnm.cu:453-495, 519-550): the same windows over the
+        // first and the last n_hemi edges of each window FC
+        for (int h = 0; h < 2 && c.save_hemis && x->n_hemi > 1; ++h) {
+            const int col0 = h == 0 ? 0 : x->np - x->n_hemi;
+            const long long rows = (long long)x->B * x->nw;
+            if (x->fc64) {
+                HIPCHK(launch_row_moments<double>((const double*)x->wfc, rows, x->n_hemi, x->wmean_h[h],
+                                                  x->winv_h[h], st, x->np, col0));
+                HIPCHK((launch_gram_fcd<double, double, double>((const double*)x->wfc, x->B, x->nw, x->np,
+                                                                x->wmean_h[h], x->winv_h[h],
+                                                                (double*)x->fcd_h[h], st, col0, x->n_hemi)));
+            } else {
+                HIPCHK(launch_row_moments<float>((const float*)x->wfc, rows, x->n_hemi, x->wmean_h[h],
+                                                 x->winv_h[h], st, x->np, col0));
+                if (x->fcbf16)
+                    HIPCHK(launch_gram_fcd_bf16((const float*)x->wfc, x->B, x->nw, x->np, x->wmean_h[h],
+                                                x->winv_h[h], (float*)x->fcd_h[h], x->fcsplit, st, col0,
+                                                x->n_hemi));
+                else
+                    HIPCHK((launch_gram_fcd<float, float, float>((const float*)x->wfc, x->B, x->nw, x->np,
+                                                                 x->wmean_h[h], x->winv_h[h],
+                                                                 (float*)x->fcd_h[h], st, col0, x->n_hemi)));
+            }
+        }
     }
     if (timed) HIPCHK(hipEventRecord(x->ev[4], st));
     if (!c.sim_only && !c.no_fcd) {
@@ -1042,6 +1079,26 @@ int bnm_ctx_get_outputs(bnm_ctx* x, double* corr, double* pen, double* wIE_fic, 
     if (!x->cfg.sim_only) {
         if (fc && (r = fetch(fc, x->fc, B * x->np, x->fc64))) return r;
         if (fcd && !x->cfg.no_fcd && (r = fetch(fcd, x->fcd, B * x->nwp, x->fc64))) return r;
+    }
+    return BNM_OK;
+}
+
+int bnm_ctx_get_hemi_fcd(bnm_ctx* x, double* fcd_left, double* fcd_right) {
+    if (!x) return fail(BNM_E_ARG, "null context");
+    if (!x->cfg.save_hemis || x->cfg.sim_only || x->cfg.no_fcd || !x->fcd_h[0])
+        return fail(BNM_E_ARG, "the context was not created with save_hemis (and FCD)");
+    if (x->n_hemi < 2) return fail(BNM_E_ARG, "a hemisphere needs at least 3 nodes");
+    const size_t n = (size_t)x->B * x->nwp;
+    double* dst[2] = {fcd_left, fcd_right};
+    for (int h = 0; h < 2; ++h) {
+        if (!dst[h]) continue;
+        if (x->fc64) {
+            HIPCHK(hipMemcpy(dst[h], x->fcd_h[h], n * sizeof(double), hipMemcpyDeviceToHost));
+        } else {
+            std::vector<float> t(n);
+            HIPCHK(hipMemcpy(t.data(), x->fcd_h[h], n * sizeof(float), hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < n; ++i) dst[h][i] = t[i];
+        }
     }
     return BNM_OK;
 }

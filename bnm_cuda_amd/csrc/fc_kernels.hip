@@ -68,12 +68,12 @@ __global__ __launch_bounds__(256) void k_seg_moments(const T* __restrict__ bold,
 
 // ---- row moments (window-FC rows over the edge axis) -------------------------
 template <class T>
-__global__ __launch_bounds__(256) void k_row_moments(const T* __restrict__ x, int len,
-                                                     double* __restrict__ mean,
+__global__ __launch_bounds__(256) void k_row_moments(const T* __restrict__ x, int len, int stride,
+                                                     int offset, double* __restrict__ mean,
                                                      double* __restrict__ inv_ssd) {
     __shared__ double red[16];
     const size_t row = blockIdx.x;
-    const T* r = x + row * len;
+    const T* r = x + row * stride + offset;
     KSum s;
     for (int i = threadIdx.x; i < len; i += blockDim.x) s.add(to_d(r[i]));
     const double tot = block_sum(s.s, red) + block_sum(s.c, red);
@@ -314,8 +314,8 @@ __global__ __launch_bounds__(256) void k_gram_fc_bf16(const Tin* __restrict__ bo
 }
 
 template <bool SPLIT>
-__global__ __launch_bounds__(256) void k_gram_fcd_bf16(const float* __restrict__ wfc, int nw, int np,
-                                                       const double* __restrict__ mean,
+__global__ __launch_bounds__(256) void k_gram_fcd_bf16(const float* __restrict__ wfc, int nw, int np, int col0,
+                                                       int K, const double* __restrict__ mean,
                                                        const double* __restrict__ inv_ssd,
                                                        float* __restrict__ fcd) {
     __shared__ __align__(16) unsigned short As[SPLIT ? 2 * TM : TM][LDB];
@@ -324,8 +324,8 @@ __global__ __launch_bounds__(256) void k_gram_fcd_bf16(const float* __restrict__
     tile_pair(blockIdx.x, ti, tj);
     const int sim = blockIdx.z;
     GramItem it;
-    it.base = wfc + (size_t)sim * nw * np;
-    it.K = np;
+    it.base = wfc + (size_t)sim * nw * np + col0;
+    it.K = K;
     it.M = nw;
     it.ldk = 1;
     it.ldm = np;
@@ -363,8 +363,8 @@ __global__ __launch_bounds__(256) void k_gram_fc(const Tin* __restrict__ bold, i
 }
 
 template <class Tin, class Tc, class Tout>
-__global__ __launch_bounds__(256) void k_gram_fcd(const Tin* __restrict__ wfc, int nw, int np,
-                                                  const double* __restrict__ mean,
+__global__ __launch_bounds__(256) void k_gram_fcd(const Tin* __restrict__ wfc, int nw, int np, int col0,
+                                                  int K, const double* __restrict__ mean,
                                                   const double* __restrict__ inv_ssd,
                                                   Tout* __restrict__ fcd) {
     __shared__ Tc As[TK][LDT];
@@ -373,8 +373,8 @@ __global__ __launch_bounds__(256) void k_gram_fcd(const Tin* __restrict__ wfc, i
     tile_pair(blockIdx.x, ti, tj);
     const int sim = blockIdx.z;
     GramItem it;
-    it.base = wfc + (size_t)sim * nw * np;
-    it.K = np;
+    it.base = wfc + (size_t)sim * nw * np + col0;
+    it.K = K;
     it.M = nw;
     it.ldk = 1;
     it.ldm = np;
@@ -397,8 +397,10 @@ hipError_t launch_seg_moments(const T* bold, int B, int N, int output_ts, int n_
 
 template <class T>
 hipError_t launch_row_moments(const T* x, long long rows, int len, double* mean, double* inv_ssd,
-                              hipStream_t st) {
-    hipLaunchKernelGGL(k_row_moments<T>, dim3((unsigned)rows), dim3(256), 0, st, x, len, mean, inv_ssd);
+                              hipStream_t st, int stride, int offset) {
+    if (stride < 0) stride = len;
+    hipLaunchKernelGGL(k_row_moments<T>, dim3((unsigned)rows), dim3(256), 0, st, x, len, stride, offset, mean,
+                       inv_ssd);
     return hipGetLastError();
 }
 
@@ -416,10 +418,11 @@ hipError_t launch_gram_fc(const Tin* bold, int B, int N, int output_ts, int n_se
 
 template <class Tin, class Tc, class Tout>
 hipError_t launch_gram_fcd(const Tin* wfc, int B, int nw, int np, const double* mean,
-                           const double* inv_ssd, Tout* fcd, hipStream_t st) {
+                           const double* inv_ssd, Tout* fcd, hipStream_t st, int col0, int K) {
+    if (K < 0) K = np;
     const int nt = (nw + TM - 1) / TM;
     hipLaunchKernelGGL((k_gram_fcd<Tin, Tc, Tout>), dim3(nt * (nt + 1) / 2, 1, B), dim3(256), 0, st,
-                       wfc, nw, np, mean, inv_ssd, fcd);
+                       wfc, nw, np, col0, K, mean, inv_ssd, fcd);
     return hipGetLastError();
 }
 
@@ -446,20 +449,21 @@ template hipError_t launch_gram_fc_bf16<double>(const double*, int, int, int, in
                                                 float*, bool, hipStream_t);
 
 hipError_t launch_gram_fcd_bf16(const float* wfc, int B, int nw, int np, const double* mean,
-                                const double* inv_ssd, float* fcd, bool split, hipStream_t st) {
+                                const double* inv_ssd, float* fcd, bool split, hipStream_t st, int col0, int K) {
+    if (K < 0) K = np;
     const int nt = (nw + TM - 1) / TM;
     const dim3 grid(nt * (nt + 1) / 2, 1, B);
     if (split)
-        hipLaunchKernelGGL(k_gram_fcd_bf16<true>, grid, dim3(256), 0, st, wfc, nw, np, mean, inv_ssd, fcd);
+        hipLaunchKernelGGL(k_gram_fcd_bf16<true>, grid, dim3(256), 0, st, wfc, nw, np, col0, K, mean, inv_ssd, fcd);
     else
-        hipLaunchKernelGGL(k_gram_fcd_bf16<false>, grid, dim3(256), 0, st, wfc, nw, np, mean, inv_ssd, fcd);
+        hipLaunchKernelGGL(k_gram_fcd_bf16<false>, grid, dim3(256), 0, st, wfc, nw, np, col0, K, mean, inv_ssd, fcd);
     return hipGetLastError();
 }
 
 #define BNM_INST_MOM(T)                                                                           \
     template hipError_t launch_seg_moments<T>(const T*, int, int, int, int, const int*, const int*, \
                                               double*, double*, hipStream_t);                     \
-    template hipError_t launch_row_moments<T>(const T*, long long, int, double*, double*, hipStream_t);
+    template hipError_t launch_row_moments<T>(const T*, long long, int, double*, double*, hipStream_t, int, int);
 BNM_INST_MOM(float)
 BNM_INST_MOM(double)
 
@@ -474,7 +478,7 @@ BNM_INST_FC(float, float)
 
 #define BNM_INST_FCD(Tc)                                                                          \
     template hipError_t launch_gram_fcd<Tc, Tc, Tc>(const Tc*, int, int, int, const double*,      \
-                                                    const double*, Tc*, hipStream_t);
+                                                    const double*, Tc*, hipStream_t, int, int);
 BNM_INST_FCD(double)
 BNM_INST_FCD(float)
 

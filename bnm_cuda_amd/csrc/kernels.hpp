@@ -78,7 +78,7 @@ hipError_t launch_gram_fc(const Tin* bold, int B, int N, int output_ts, int n_se
                           hipStream_t st);
 template <class Tin, class Tc, class Tout>
 hipError_t launch_gram_fcd(const Tin* wfc, int B, int nw, int np, const double* mean,
-                           const double* inv_ssd, Tout* fcd, hipStream_t st);
+                           const double* inv_ssd, Tout* fcd, hipStream_t st, int col0 = 0, int K = -1);
 // BF16-operand variants (FP32 accumulation, FP32 outputs); split = hi/lo
 // three-product form
 template <class Tin>
@@ -87,11 +87,12 @@ hipError_t launch_gram_fc_bf16(const Tin* bold, int B, int N, int output_ts, int
                                const double* inv_ssd, const int* pidx, int np, float* fc, float* wfc,
                                bool split, hipStream_t st);
 hipError_t launch_gram_fcd_bf16(const float* wfc, int B, int nw, int np, const double* mean,
-                                const double* inv_ssd, float* fcd, bool split, hipStream_t st);
+                                const double* inv_ssd, float* fcd, bool split, hipStream_t st, int col0 = 0,
+                                int K = -1);
 // Row moments of the window-FC matrix (mean, 1/ssd over the np edges).
 template <class T>
 hipError_t launch_row_moments(const T* x, long long rows, int len, double* mean, double* inv_ssd,
-                              hipStream_t st);
+                              hipStream_t st, int stride = -1, int offset = 0);
 
 // ---- fit (gof_kernels.hip) ---------------------------------------------------
 struct GofArgs {

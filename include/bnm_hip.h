@@ -94,6 +94,9 @@ typedef struct bnm_config {
     int use_graph;             /* capture the whole run in a hipGraph at the first
                                   run and replay it (per-phase timings are then
                                   not available, only the total)                 */
+    int save_hemis;            /* also compute left / right FCD over the first /
+                                  last (N/2)(N/2-1)/2 edges of each window FC
+                                  (bnm.cu:453-495, 519-550)                       */
 } bnm_config;
 
 /* Fills `cfg` with the reference defaults (constants.h:93-116), honouring the
@@ -190,6 +193,12 @@ int bnm_ctx_get_outputs(bnm_ctx* ctx, double* corr, double* fic_penalties, doubl
                         double* BOLD, double* S_E, double* I_E, double* r_E, double* S_I,
                         double* I_I, double* r_I, int* FIC_failed, int* fic_n_trials,
                         double* fc_trils, double* fcd_trils);
+/* Left and right FCD [B*n_window_pairs] of the last run (save_hemis): the
+ * reference's save_hemis outputs L_fcd_trils / R_fcd_trils (bnm.cu:498-550),
+ * with n_pairs_hemi = (N/2)(N/2-1)/2.  With exc_interhemispheric on, the
+ * first and last n_pairs_hemi edges are exactly the left-left and
+ * right-right pairs. */
+int bnm_ctx_get_hemi_fcd(bnm_ctx* ctx, double* fcd_left, double* fcd_right);
 /* FC, FCD and fit of a given BOLD array (no integration): bnm_ctx_set_bold
  * copies BOLD [B*output_ts*N] (time-major per simulation, the layout
  * bnm_ctx_get_outputs returns) into the context, bnm_ctx_analyze computes

@@ -382,3 +382,28 @@ def test_hybrid_resident_sc_kernel_vs_f64(monkeypatch, N_big, ks):
           f"d rE {d_re:.3g}, d FC {d_fc:.3g}")
     np.testing.assert_array_equal(a["fic_trials"], b["fic_trials"])
     assert rel <= 1e-4 and d_re <= 0.05 and d_fc <= 1e-3
+
+
+@pytest.mark.parametrize("exc", [1, 0])
+def test_hemisphere_fcd_matches_window_fc_subsets(oracle_mod, case, ref, exc):
+    """save_hemis: left / right FCD = Pearson between windows over the first /
+    last (N/2)(N/2-1)/2 edges of the window FC (the reference's save_hemis
+    kernels), checked against numpy on the oracle's window FC."""
+    cfg = _cfg(case, bnm.PREC_F64, bnm.KERNEL_FAST, save_hemis=1, exc_interhemispheric=exc)
+    np_ = bnm.n_pairs(N, exc)
+    emp_fc = np.resize(case["emp_fc"], np_)
+    with bnm.Context(cfg) as c:
+        c.set_empirical(emp_fc, case["emp_fcd"])
+        c.set_bold(ref["bold"])
+        c.analyze()
+        left, right = c.hemi_fcd()
+        g = c.outputs()
+    fcd_o, wfc = oracle_mod.fcd_tril(ref["bold"][0][NVR:], STEP, WS, bool(exc), want_window_fc=True)
+    nh = (N // 2) * (N // 2 - 1) // 2
+    il = np.tril_indices(wfc.shape[0], -1)
+    for got, cols in ((left, slice(0, nh)), (right, slice(np_ - nh, np_))):
+        want = np.corrcoef(wfc[:, cols])[il]
+        np.testing.assert_allclose(got[0], want, atol=1e-10)
+    np.testing.assert_allclose(g["fcd"][0], fcd_o, atol=1e-10)
+    if exc:
+        assert 2 * nh == np_
