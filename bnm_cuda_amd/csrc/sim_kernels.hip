@@ -936,12 +936,6 @@ FastShape hyb_shape(int N, int KS, size_t lds_bytes) {
 template <int RR>
 static void launch_hyb(const float* SCc, const SimArgs& a, const FastShape& s, hipStream_t st) {
     constexpr int MAXT = RR <= 96 ? 512 : 256;
-    static bool attr = false;   // allow more than 64 KB of dynamic LDS
-    if (!attr) {
-        (void)hipFuncSetAttribute((const void*)k_sim_hyb<RR, MAXT>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  160 * 1024);
-        attr = true;
-    }
     hipLaunchKernelGGL((k_sim_hyb<RR, MAXT>), dim3(a.B), dim3(s.threads), s.smem, st, SCc, a, s.NW, s.KS,
                        s.kc, s.SS);
 }
