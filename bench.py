@@ -119,6 +119,25 @@ def cpu_baseline(SC, params, noise, N, T_sample, n_sims, threads):
     return wall, integrated
 
 
+def copy_calibration():
+    """A plain device copy of known size (1 GiB read + 1 GiB written per
+    launch, 5 launches): the reference point for scaling the FETCH_SIZE /
+    WRITE_SIZE counters of the kernels into HBM bytes."""
+    import torch
+    n = 1 << 28
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 5
+    print(json.dumps({"workload": "copy", "bytes_read_per_launch": 4 * n, "bytes_written_per_launch": 4 * n,
+                      "ms_per_launch": round(dt * 1e3, 3),
+                      "gb_s": round(8 * n / dt / 1e9, 1)}), flush=True)
+
+
 def surrogate_experiment(args):
     """Surrogate-guided CMA-ES (learned FIC initialiser + fit pre-screening)
     against plain CMA-ES, both through the run_cmaes program on the same
@@ -206,7 +225,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-sample-ms", type=int, default=60_000)
     ap.add_argument("--cpu-sims", type=int, default=16)
-    ap.add_argument("--workload", default="pop128", choices=["pop128", "surrogate", "uneven", "fcd400"],
+    ap.add_argument("--sims", type=int, default=128, help="population size (profiling variants)")
+    ap.add_argument("--precision", default="f32", choices=["f32", "bf16", "bf16x3"],
+                    help="coupling precision of the timed path (profiling variants)")
+    ap.add_argument("--kernel", default="auto", choices=["auto", "fast", "mfma"],
+                    help="integration kernel family of the timed path (profiling variants)")
+    ap.add_argument("--workload", default="pop128", choices=["pop128", "surrogate", "uneven", "fcd400", "copy"],
                     help="pop128: the headline line; surrogate: CMA-ES with vs without the surrogate; "
                          "uneven: four populations (N 68-400) in one graph launch; fcd400: FC/FCD stress")
     ap.add_argument("--scale", type=float, default=1.0, help="duration scale (uneven workload)")
@@ -227,13 +251,16 @@ def main():
     if args.workload == "surrogate":
         surrogate_experiment(args)
         return
+    if args.workload == "copy":
+        copy_calibration()
+        return
     if args.workload in ("uneven", "fcd400"):
         from bnm_cuda_amd import experiments as X
         res = X.uneven_batch(durations_scale=args.scale) if args.workload == "uneven" else X.fcd_stress()
         print(json.dumps(res), flush=True)
         return
 
-    N, B, T = 100, 128, args.duration_ms
+    N, B, T = 100, args.sims, args.duration_ms
     log(f"[bench] generating inputs: N={N} B={B} T={T} ms, noise {(T + 1) * 20 * N / 1e6:.0f} M draws")
     t0 = time.perf_counter()
     SC, g, wEE, wEI, noise = make_inputs(N, B, T)
@@ -243,7 +270,9 @@ def main():
     params = (g, wEE, wEI)
 
     # ---- timed FP32 batched path --------------------------------------------
-    cfg32 = cfg_for(N, B, T, bnm.PREC_F32, bnm.KERNEL_AUTO)
+    prec = {"f32": bnm.PREC_F32, "bf16": bnm.PREC_BF16, "bf16x3": bnm.PREC_BF16X3}[args.precision]
+    kern = {"auto": bnm.KERNEL_AUTO, "fast": bnm.KERNEL_FAST, "mfma": bnm.KERNEL_MFMA}[args.kernel]
+    cfg32 = cfg_for(N, B, T, prec, kern)
     wall, tm, o32, kname = run_ctx(cfg32, SC, noise, emp, params, args.steps, args.warmup)
     ms_step = wall / args.steps * 1e3
     nominal = B * T / 1000.0
@@ -321,9 +350,10 @@ def main():
         "metric": "simulated sec/s (N=100, 128 sims, FP32 vs FP64) + FC/FCD-KS error vs FP64",
         "value": round(value, 1), "unit": "simulated s / wall s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 2),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if args.precision == "f32" else f"f32 state, {args.precision} coupling",
         "data": "synthetic",
-        "config": {"workload": "pop128: 128 sims x 450 s, N=100, TR 1 s, FCD window 30 step 5, "
+        "config": {"workload": f"pop{B}: {B} sims x {T / 1000:g} s, N=100, TR 1 s, FCD window 30 step 5, "
                                "analytical+numerical FIC, full-length noise",
                    "nodes": N, "n_sims": B, "duration_s": T / 1000, "kernel": kname,
                    "fic_trials_per_step": int(trials.sum()),
