@@ -38,6 +38,16 @@ from bnm_cuda_amd import workloads as W  # noqa: E402
 PEAK_FP32_TFLOPS = 157.3   # MI355X dense FP32 matrix peak
 PEAK_HBM_GBS = 8000.0
 
+# HBM bytes per launch of the timed integration kernel on this workload,
+# from two rocprofv3 --pmc passes over `bench.py --steps 1 --warmup 0
+# --no-fp64 --no-cpu` (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each
+# scaled by what the counter reads for a 1 GiB device copy (FETCH_SIZE reads
+# 0.5x, WRITE_SIZE 1.0x): 79,063,470 KiB x 2 fetched + 26,524 KiB written.
+# Valid for the split_ei kernel family at 128 x 450 s; other shapes report
+# null.
+MEASURED_TRAFFIC = {"kernel_prefix": "split_ei_f32", "n_sims": 128, "duration_ms": 450_000,
+                    "bytes": 79_063_470 * 1024 * 2 + 26_524 * 1024}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -288,9 +298,13 @@ def main():
     # restarts) over the kernel's HIP-event time
     flops = 2.0 * N * N * 10 * (B * (T + 1) + 10_000 * int(trials.sum()))
     achieved = flops / (integ_ms * 1e-3) / 1e12
+    mt = MEASURED_TRAFFIC
+    traffic = (mt["bytes"] if kname.startswith(mt["kernel_prefix"]) and B == mt["n_sims"] and
+               T == mt["duration_ms"] else None)
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 5), "traffic": None,
-                "kernel": kname, "kernel_ms": round(integ_ms, 2)}
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 5), "traffic": traffic,
+                "kernel": kname, "kernel_ms": round(integ_ms, 2),
+                "hbm_gb_s": round(traffic / (integ_ms * 1e-3) / 1e9, 1) if traffic else None}
 
     extra = {}
     # ---- FP64 reference runs: error bars and the plain-path ratio -------------
