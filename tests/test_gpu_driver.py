@@ -132,9 +132,12 @@ def test_surrogate_mode_runs_and_reports(inputs, tmp_path):
     out = tmp_path / "surr"
     stdout = _run(d, out, {"BNM_SURROGATE": "1", "BNM_SURR_FIC_SIMS": "64", "BNM_SURR_FIC_EPOCHS": "20",
                            "BNM_SURR_FIT_STEPS": "300"})
-    assert "Surrogate FIC initialiser" in stdout and "Surrogate summary: full simulations 30" in stdout
+    # 10 candidates per generation plus the plain re-run of each generation's best
+    assert "Surrogate FIC initialiser" in stdout and "Surrogate summary: full simulations 33" in stdout
     assert len(_parse(stdout)) == (ITMAX - 1) * LAM
     prog = [p for p in os.listdir(out / "cmaes_multimaps_gpu") if p.endswith("_progress.csv")][0]
     rows = np.genfromtxt(out / "cmaes_multimaps_gpu" / prog, delimiter=",", names=True)
-    assert rows["full_sims"].tolist() == [10, 20, 30]
+    assert rows["full_sims"].tolist() == [11, 22, 33]
+    assert np.all(np.isfinite(rows["best_plain_cost"]))
+    assert stdout.count("plain re-run of the generation best") == ITMAX - 1
     assert rows["surrogate_used"].tolist()[0] == 0 and rows["surrogate_used"].max() == 1
