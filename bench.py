@@ -192,7 +192,8 @@ def surrogate_experiment(args):
 
     plain, w_plain, plain_rerun, _ = run("plain", lam, {})
     surr, w_surr, surr_rerun, surr_text = run("surrogate", args.surrogate_lambda,
-                                              {"BNM_SURROGATE": "1", "BNM_SURR_POOL": str(args.pool)})
+                                              {"BNM_SURROGATE": "1", "BNM_SURR_POOL": str(args.pool),
+                                               "BNM_SURR_FIC": str(args.surrogate_fic)})
     # every cost compared here is a plain-method simulation: the plain run's
     # own costs, and the surrogate run's plain re-runs of its generation bests
     best_plain = float(plain[-1]["best_plain_cost"])
@@ -205,7 +206,8 @@ def surrogate_experiment(args):
         "n_gpus": 1, "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"CMA-ES N=100, {T / 1000:.0f} s, 5 free parameters (G, wEE, wEI, "
                                f"weescale_0, weiscale_0), {args.generations} generations",
-                   "plain_lambda": lam, "surrogate_lambda": args.surrogate_lambda, "pool": args.pool},
+                   "plain_lambda": lam, "surrogate_lambda": args.surrogate_lambda, "pool": args.pool,
+                   "learned_fic_start": bool(args.surrogate_fic)},
         "plain": {"best_cost": best_plain, "full_sims_total": sims_plain_total,
                   "full_sims_to_best": sims_plain_first, "wall_s": round(w_plain, 1),
                   "mean_fic_restarts": float(np.mean([float(r["mean_fic_trials"]) for r in plain])),
@@ -251,6 +253,7 @@ def main():
     ap.add_argument("--generations", type=int, default=32)
     ap.add_argument("--surrogate-lambda", type=int, default=64)
     ap.add_argument("--pool", type=int, default=4)
+    ap.add_argument("--surrogate-fic", type=int, default=1, help="learned FIC start in the surrogate run")
     ap.add_argument("--seed-mw", type=int, default=1)
     ap.add_argument("--window", type=int, default=30, help="FCD window (surrogate workload)")
     ap.add_argument("--step", type=int, default=5, help="FCD window step (surrogate workload)")

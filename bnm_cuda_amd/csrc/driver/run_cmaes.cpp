@@ -14,6 +14,7 @@
 //   BNM_NOISE_CHUNKED  1: parallel chunked noise generator (a different stream)
 //   BNM_SURROGATE      1: surrogate-guided search (below)
 //   BNM_PROGRESS_CSV   1 (default): per-generation progress file next to the log
+//   BNM_SURR_FIC       0: surrogate search without the learned FIC start
 //
 // Surrogate-guided search (BNM_SURROGATE=1):
 //  * FIC initialiser: before the search, BNM_SURR_FIC_SIMS parameter sets drawn
@@ -390,7 +391,7 @@ int main(int argc, char* argv[]) {
             }
         return X;
     };
-    if (surrogate && do_fic && cfg.adjust_fic) {
+    if (surrogate && do_fic && cfg.adjust_fic && env_i("BNM_SURR_FIC", 1)) {
         // FIC-phase training set: parameter sets drawn uniformly in the search box
         const int n_train = env_i("BNM_SURR_FIC_SIMS", 8 * lambda);
         const int t_max = env_i("BNM_SURR_FIC_TRIALS", 50);
