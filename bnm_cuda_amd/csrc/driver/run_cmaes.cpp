@@ -24,8 +24,9 @@
 //    converged wIE_j minus the analytical one.  Every simulation of the search
 //    then starts FIC from analytical + predicted wIE.
 //  * fit surrogate: an MLP on the unit-box coordinates predicts
-//    (FC r, FCD KS, |mean FC difference|, FIC penalty) of a candidate.  It is
-//    trained on every simulation run so far.  From generation
+//    (FC r, FCD KS, |mean FC difference|, FIC penalty, cost) of a candidate.
+//    It is trained on the simulations of the last BNM_SURR_WINDOW
+//    generations, inputs and outputs standardised over them.  From generation
 //    BNM_SURR_WARMUP + 1 on, each generation draws BNM_SURR_POOL x lambda
 //    candidates, ranks them by predicted cost and simulates the lambda best in
 //    full; when the rank correlation of predicted and simulated cost of the
@@ -457,10 +458,18 @@ int main(int argc, char* argv[]) {
              n_train, rows / nodes, tr_sum / n_train, fic_train_sim_s, rows, losses.front(), losses.back(),
              std::chrono::duration<double>(Clock::now() - t0).count());
     }
+    // fit surrogate: unit-box coordinates -> (FC r, FCD KS, |mean FC diff|,
+    // FIC penalty, cost); trained on the last `window` generations, inputs
+    // and outputs standardised over that window (so the network resolves
+    // the region the search has narrowed to), ranked by the cost output
+    constexpr int KF = 5;
     bnm_mlp* fit_net = nullptr;
-    std::vector<float> fit_X, fit_Y;   // unit-box coordinates -> (r, KS, |dmean|, fic penalty)
+    std::vector<float> fit_X, fit_Y;
+    std::vector<int> fit_gen;            // generation of each row
+    Scaler fit_xs, fit_ys;
+    const int window = std::max(1, env_i("BNM_SURR_WINDOW", 6));
     if (surrogate) {
-        fit_net = bnm_mlp_create(D, mlp_hidden, mlp_depth, 4, 1, 13, &st);
+        fit_net = bnm_mlp_create(D, mlp_hidden, mlp_depth, KF, 1, 13, &st);
         if (!fit_net) die(std::string("MLP: ") + bnm_mlp_last_error());
     }
 
@@ -520,13 +529,11 @@ int main(int argc, char* argv[]) {
             const int n_pool = pool_factor * lambda;
             std::vector<double> pool((size_t)n_pool * D);
             check(bnm_cmaes_ask_pool(es, n_pool, 7919u * it + (unsigned)SeedMW, pool.data()), "ask_pool");
-            std::vector<float> px(pool.begin(), pool.end()), py((size_t)n_pool * 4);
+            std::vector<float> px(pool.begin(), pool.end()), py((size_t)n_pool * KF);
+            fit_xs.apply(px, D);
             check(bnm_mlp_predict(fit_net, px.data(), n_pool, py.data()), "surrogate predict");
             std::vector<double> c(n_pool);
-            for (int i = 0; i < n_pool; ++i) {
-                const float* q = &py[(size_t)i * 4];
-                c[i] = 1 - q[0] + q[1] + q[2] + (do_fic ? q[3] : 0.0);
-            }
+            for (int i = 0; i < n_pool; ++i) c[i] = fit_ys.mu[KF - 1] + fit_ys.sd[KF - 1] * py[(size_t)i * KF + KF - 1];
             std::vector<int> idx(n_pool);
             std::iota(idx.begin(), idx.end(), 0);
             std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return c[a] < c[b]; });
@@ -590,13 +597,26 @@ int main(int argc, char* argv[]) {
                 fit_Y.push_back((float)ks);
                 fit_Y.push_back((float)(ok ? dm : half));
                 fit_Y.push_back((float)(do_fic ? pen[b] : 0.0));
+                fit_Y.push_back((float)raw[b]);
+                fit_gen.push_back(it);
                 actual[b] = raw[b];
             }
             if (used_surr) last_rho = spearman(pred_cost, actual);
-            const int n = (int)(fit_Y.size() / 4);
-            check(bnm_mlp_set_data(fit_net, fit_X.data(), fit_Y.data(), n), "surrogate data");
+            // training window: the last `window` generations
+            std::vector<float> wx, wy;
+            for (size_t i = 0; i < fit_gen.size(); ++i)
+                if (fit_gen[i] > it - window) {
+                    wx.insert(wx.end(), &fit_X[i * D], &fit_X[i * D] + D);
+                    wy.insert(wy.end(), &fit_Y[i * KF], &fit_Y[i * KF] + KF);
+                }
+            const int n = (int)(wy.size() / KF);
+            fit_xs.fit(wx, n, D);
+            fit_ys.fit(wy, n, KF);
+            fit_xs.apply(wx, D);
+            fit_ys.apply(wy, KF);
+            check(bnm_mlp_set_data(fit_net, wx.data(), wy.data(), n), "surrogate data");
             const int steps = env_i("BNM_SURR_FIT_STEPS", 1500);
-            const int batch = std::min(n, 256);
+            const int batch = std::min(n, 128);
             const int epochs = std::max(1, steps / std::max(1, n / batch));
             check(bnm_mlp_train(fit_net, epochs, batch, 1e-3f, 17u + it, nullptr), "surrogate train");
             if (!used_surr) last_rho = 1.0;   // the first fit enables pre-screening
