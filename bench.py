@@ -252,8 +252,8 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0, help="duration scale (uneven workload)")
     ap.add_argument("--generations", type=int, default=32)
     ap.add_argument("--surrogate-lambda", type=int, default=64)
-    ap.add_argument("--pool", type=int, default=4)
-    ap.add_argument("--surrogate-fic", type=int, default=1, help="learned FIC start in the surrogate run")
+    ap.add_argument("--pool", type=int, default=8)
+    ap.add_argument("--surrogate-fic", type=int, default=0, help="learned FIC start in the surrogate run")
     ap.add_argument("--seed-mw", type=int, default=1)
     ap.add_argument("--window", type=int, default=30, help="FCD window (surrogate workload)")
     ap.add_argument("--step", type=int, default=5, help="FCD window step (surrogate workload)")
