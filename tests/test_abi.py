@@ -76,6 +76,11 @@ def test_default_config_follows_reference_env(monkeypatch):
     dict(time_steps=20_000),         # fewer than 2 volumes after the discard
     dict(window_size=58),            # one window only
     dict(n_sims=0),
+    dict(nodes=1026),                # above the 1024-node limit
+    dict(bold_tr=0),
+    dict(window_step=0),
+    dict(noise_period_ms=0),
+    dict(i_sampling_start=9000, i_sampling_end=1000),
 ])
 def test_finalize_rejects_bad_configs(bad):
     kw = dict(nodes=100, n_sims=4, time_steps=60_000, bold_tr=1000, window_size=10, window_step=2)

@@ -407,3 +407,56 @@ def test_hemisphere_fcd_matches_window_fc_subsets(oracle_mod, case, ref, exc):
     np.testing.assert_allclose(g["fcd"][0], fcd_o, atol=1e-10)
     if exc:
         assert 2 * nh == np_
+
+
+@pytest.mark.parametrize("N_e,prec,T_e", [(4, bnm.PREC_F64, 14_000), (4, bnm.PREC_F32, 14_000),
+                                          (130, bnm.PREC_F32, 14_000), (1000, bnm.PREC_F64, 2_000),
+                                          (1000, bnm.PREC_F32, 2_000)])
+def test_size_extremes_match_oracle(oracle_mod, N_e, prec, T_e):
+    """Smallest and largest node counts (and N just above the register-resident
+    limit) through the autotuned kernels against the oracle: 14 s runs with
+    FIC adjustment and the FC / FCD fit, and at N = 1000 (where the oracle is
+    slow) 2 s of integration without adjustment."""
+    B_e = 2
+    fit = T_e >= 10_000
+    SC = W.synthetic_sc(N_e, 0.3 if N_e > 4 else 0.9, 31)
+    rng = np.random.default_rng(32)
+    G = np.array([0.8, 1.6])
+    wEE = np.full((B_e, N_e), 0.2) + 0.01 * rng.normal(size=(B_e, N_e))
+    wEI = np.full((B_e, N_e), 0.15)
+    noise = bnm.make_noise_chunked(33, (T_e + 1) * 20 * N_e)
+    np_ = bnm.n_pairs(N_e, False)
+    emp_fc = rng.uniform(-0.2, 0.8, np_)
+    emp_fcd = rng.uniform(-0.5, 1, 20)
+    cfg = bnm.default_config(nodes=N_e, n_sims=B_e, time_steps=T_e, bold_tr=1000, n_vols_remove=0,
+                             window_size=4, window_step=2, exc_interhemispheric=0, noise_period_ms=T_e + 1,
+                             max_fic_trials=2, adjust_fic=int(fit), sim_only=int(not fit), precision=prec,
+                             fc_precision=bnm.FC_F64 if prec == bnm.PREC_F64 else bnm.FC_F32)
+    print("case: size extreme N", N_e, "precision", prec, flush=True)
+    with bnm.Context(cfg) as c:
+        c.set_sc(SC); c.set_noise(noise)
+        if fit:
+            c.set_empirical(emp_fc, emp_fcd)
+        c.set_params(G, wEE, wEI)
+        c.run()
+        name = c.kernel_name()
+        g = c.outputs(fc=fit, fcd=fit)
+    an, _ = bnm.analytical_fic(SC, G, wEE, wEI)
+    o = oracle_mod.run_simulations(SC, G, wEE, wEI, noise_arr=noise, noise_period_ms=T_e + 1, time_steps=T_e,
+                                   bold_tr=1000, n_vols_remove=0, window_size=4, window_step=2, emp_fc=emp_fc,
+                                   emp_fcd=emp_fcd, analytical_init=False, wIE_fic=an, max_fic_trials=2,
+                                   adjust_fic=fit, sim_only=not fit, exc=False, n_threads=2)
+    rel = _bold_rel(g["bold"][:, 1:], o["bold"][:, 1:])     # BOLD(t=0) is exactly 0
+    print(name, "trials", g["fic_trials"], o["fic_trials"], "BOLD rel", rel)
+    np.testing.assert_array_equal(g["fic_trials"], o["fic_trials"])
+    if prec == bnm.PREC_F64:
+        assert rel <= 1e-9
+        np.testing.assert_allclose(g["ext"], o["ext"], rtol=1e-9, atol=1e-12)
+        if fit:
+            np.testing.assert_allclose(g["fc"], o["fc"], atol=1e-8)
+            np.testing.assert_allclose(g["corr"], o["corr"], atol=1e-8)
+    else:
+        assert rel <= 1e-4
+        assert np.abs(g["ext"][:, 2] - o["ext"][:, 2]).max() <= 0.05
+        if fit:
+            assert np.abs(g["fc"] - o["fc"]).max() <= 1e-3
