@@ -661,7 +661,7 @@ __global__ __launch_bounds__(MAXT) void k_sim_hyb(const float* __restrict__ SCc,
             w[4 * q + 3] = __uint_as_float(v[3]);
         }
     };
-    float wr[RR];
+    float wr[RR > 0 ? RR : 1];
 #pragma unroll
     for (int c = 0; c < RR / CH; ++c) chunk(k0 / CH + c, wr + c * CH);
     for (int c = 0; c < LL / CH; ++c) {
@@ -920,7 +920,7 @@ FastShape hyb_shape(int N, int KS, size_t lds_bytes) {
     FastShape s = stream_shape(N, KS, sizeof(float));
     // RR: the register part the workgroup size leaves room for (one or two
     // waves per SIMD), capped by the lane's k-range
-    s.KC = s.threads <= 256 ? 160 : 96;
+    s.KC = s.threads <= 256 ? 160 : s.threads <= 512 ? 96 : 0;
     while (s.KC > s.kc) s.KC -= 16;
     const size_t base = (2 * (size_t)KS * s.kc + (size_t)KS * s.NW * 64) * sizeof(float);
     int LL = 0;
@@ -928,14 +928,14 @@ FastShape hyb_shape(int N, int KS, size_t lds_bytes) {
     LL = std::min(LL, s.kc - s.KC) / 16 * 16;   // whole chunks
     s.SS = LL;
     s.smem = base + (size_t)LL * s.threads * sizeof(float);
-    s.ok = s.threads <= 512 && s.smem <= lds_bytes && (s.kc - s.KC - LL) % 16 == 0 && s.KC % 16 == 0 &&
-           (s.KC == 160 || s.KC == 96 || s.KC == 64 || s.KC == 32 || s.KC == 16);
+    s.ok = s.threads <= 1024 && s.smem <= lds_bytes && (s.kc - s.KC - LL) % 16 == 0 && s.KC % 16 == 0 &&
+           (s.KC == 160 || s.KC == 96 || s.KC == 64 || s.KC == 32 || s.KC == 16 || s.KC == 0);
     return s;
 }
 
 template <int RR>
 static void launch_hyb(const float* SCc, const SimArgs& a, const FastShape& s, hipStream_t st) {
-    constexpr int MAXT = RR <= 96 ? 512 : 256;
+    constexpr int MAXT = RR == 0 ? 1024 : RR <= 96 ? 512 : 256;
     hipLaunchKernelGGL((k_sim_hyb<RR, MAXT>), dim3(a.B), dim3(s.threads), s.smem, st, SCc, a, s.NW, s.KS,
                        s.kc, s.SS);
 }
@@ -944,6 +944,7 @@ hipError_t launch_sim_hyb(const float* SCc, const SimArgs& a, int KS, size_t lds
     const FastShape s = hyb_shape(a.N, KS, lds_bytes);
     if (!s.ok) return hipErrorInvalidValue;
     switch (s.KC) {
+        case 0: launch_hyb<0>(SCc, a, s, st); break;
         case 16: launch_hyb<16>(SCc, a, s, st); break;
         case 32: launch_hyb<32>(SCc, a, s, st); break;
         case 64: launch_hyb<64>(SCc, a, s, st); break;

@@ -343,7 +343,7 @@ def test_n400_streamed_f32_vs_f64():
     assert np.abs(a["fc"] - b["fc"]).max() <= 1e-3
 
 
-@pytest.mark.parametrize("N_big,ks", [(200, 1), (200, 2), (400, 1)])
+@pytest.mark.parametrize("N_big,ks", [(200, 1), (200, 2), (200, 4), (400, 1), (400, 2)])
 def test_hybrid_resident_sc_kernel_vs_f64(monkeypatch, N_big, ks):
     """FP32 kernel with register- and LDS-resident SC and the rest streamed
     (chunked SC layout, buffer loads) against the FP64 streamed kernel on
