@@ -73,6 +73,7 @@ struct bnm_ctx {
     bool fc64 = true;     // Gram precision
     bool fcbf16 = false;  // BF16 Gram operands (FP32 accumulation and storage)
     bool fcsplit = false; // ... as hi/lo pairs (three products)
+    bool fcd_direct = false;
     bool have_bold = false;  // BOLD holds a run's or a caller's series
     hipStream_t st = nullptr;
     double* SC64 = nullptr;
@@ -91,6 +92,10 @@ struct bnm_ctx {
     int *seg_start = nullptr, *seg_len = nullptr, *pidx = nullptr;
     double *mean = nullptr, *inv = nullptr, *wmean = nullptr, *winv = nullptr;
     void *fc = nullptr, *wfc = nullptr, *fcd = nullptr;
+    // z-scored window-FC rows for the FP32 / BF16 FCD Gram ([B*nw][np rounded
+    // up to 64]; zlo: BF16 low parts for the hi/lo split)
+    void* wz = nullptr;
+    unsigned short* wzlo = nullptr;
     // hemisphere FCDs (save_hemis): first / last n_hemi edges of each window FC
     void *fcd_h[2] = {nullptr, nullptr};
     double *wmean_h[2] = {nullptr, nullptr}, *winv_h[2] = {nullptr, nullptr};
@@ -263,7 +268,7 @@ void bnm_ctx_destroy(bnm_ctx* x) {
     void* bufs[] = {x->SC64, x->SC32, x->SCt64, x->SCt32, x->SCc32, x->noise, x->G, x->wEE, x->wEI, x->wIE, x->wIEfic,
                     x->wIEscratch, x->bold, x->ext, x->failed, x->trials, x->unstable,
                     x->seg_start, x->seg_len, x->pidx, x->mean, x->inv, x->wmean, x->winv, x->fc,
-                    x->wfc, x->fcd, x->emp_c, x->ks_u, x->ks_fe, x->bins, x->nanf, x->corr, x->pen,
+                    x->wfc, x->wz, x->wzlo, x->fcd, x->emp_c, x->ks_u, x->ks_fe, x->bins, x->nanf, x->corr, x->pen,
                     x->fcd_h[0], x->fcd_h[1], x->wmean_h[0], x->wmean_h[1], x->winv_h[0], x->winv_h[1]};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
@@ -334,6 +339,11 @@ static int ctx_alloc(bnm_ctx* x) {
             HIPCHK(hipMalloc(&x->wfc, B * (size_t)x->nw * x->np * fsz));
             if ((r = dalloc(&x->wmean, B * x->nw))) return r;
             if ((r = dalloc(&x->winv, B * x->nw))) return r;
+            if (!x->fc64 && !x->fcd_direct) {
+                const size_t zn = B * (size_t)x->nw * ((x->np + 63) / 64 * 64);
+                HIPCHK(hipMalloc(&x->wz, zn * (x->fcbf16 ? 2 : 4)));
+                if (x->fcsplit) HIPCHK(hipMalloc(&x->wzlo, zn * 2));
+            }
             HIPCHK(hipMalloc(&x->fcd, B * (size_t)x->nwp * fsz));
             if (c.save_hemis) {
                 x->n_hemi = (x->N / 2) * (x->N / 2 - 1) / 2;
@@ -429,6 +439,9 @@ bnm_ctx* bnm_ctx_create(const bnm_config* cfg, int* status) {
     x->fc64 = c.fc_precision == BNM_FC_F64;
     x->fcbf16 = c.fc_precision == BNM_FC_BF16 || c.fc_precision == BNM_FC_BF16X3;
     x->fcsplit = c.fc_precision == BNM_FC_BF16X3;
+    // BNM_FCD_DIRECT=1: z-score inside the FCD Gram's tile loader instead of
+    // a separate pass (no z buffer; slower, kept for comparison)
+    x->fcd_direct = env_int("BNM_FCD_DIRECT", 0) != 0;
     if (!c.sim_only) {
         x->np = bnm_n_pairs(c.nodes, c.exc_interhemispheric);
         x->nw = c.no_fcd ? 0 : bnm_n_windows(&c);
@@ -832,7 +845,10 @@ static int enqueue_run(bnm_ctx* x, hipStream_t st, bool timed, bool analyze_only
         } else {
             HIPCHK(launch_row_moments<float>((const float*)x->wfc, (long long)x->B * x->nw, x->np,
                                              x->wmean, x->winv, st));
-            if (x->fcbf16)
+            if (x->wz)
+                HIPCHK(launch_fcd_z((const float*)x->wfc, x->B, x->nw, x->np, x->wmean, x->winv, x->wz, x->wzlo,
+                                    x->fcsplit ? 2 : x->fcbf16 ? 1 : 0, (float*)x->fcd, st));
+            else if (x->fcbf16)
                 HIPCHK(launch_gram_fcd_bf16((const float*)x->wfc, x->B, x->nw, x->np, x->wmean, x->winv,
                                             (float*)x->fcd, x->fcsplit, st));
             else

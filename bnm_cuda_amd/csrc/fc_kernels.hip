@@ -384,6 +384,193 @@ __global__ __launch_bounds__(256) void k_gram_fcd(const Tin* __restrict__ wfc, i
     gram_tile<Tin, Tc, Tout>(it, ti, tj, nullptr, As, Bs);
 }
 
+// ---- FCD Gram over pre-z-scored rows -----------------------------------------
+// The window-FC matrix (per simulation nw rows of np edges) is z-scored once
+// per row into Z [rows][npp] (npp = np rounded up to 64, zero padded): FP32,
+// or BF16 hi (+ lo = bf16(z - hi) for the split form).  The Gram tiles then
+// move 16-byte vectors from global memory to LDS with no per-element
+// arithmetic, and the MFMA issue rate bounds the loop instead of the loader.
+namespace {
+constexpr int ZT = 64;         // output tile edge
+constexpr int ZK32 = 32;       // K chunk, FP32 operands
+constexpr int ZKB = 64;        // K chunk, BF16 operands
+constexpr int ZLD32 = ZK32 + 4;   // floats per staged row (bank offset)
+constexpr int ZLDB = ZKB + 8;     // u16 per staged row
+}  // namespace
+
+template <int MODE>   // 0: FP32, 1: BF16, 2: BF16 hi + lo
+__global__ __launch_bounds__(256) void k_zscore_rows(const float* __restrict__ x, int np, int npp,
+                                                     const double* __restrict__ mean,
+                                                     const double* __restrict__ inv_ssd, void* __restrict__ zhi,
+                                                     unsigned short* __restrict__ zlo) {
+    const size_t row = blockIdx.x;
+    const double m = mean[row], is = inv_ssd[row];
+    const float* r = x + row * np;
+    for (int i = threadIdx.x; i < npp; i += blockDim.x) {
+        const float z = i < np ? (float)(((double)r[i] - m) * is) : 0.0f;
+        if constexpr (MODE == 0) {
+            reinterpret_cast<float*>(zhi)[row * npp + i] = z;
+        } else {
+            const unsigned short h = bf16_bits(z);
+            reinterpret_cast<unsigned short*>(zhi)[row * npp + i] = h;
+            if constexpr (MODE == 2) zlo[row * npp + i] = bf16_bits(z - __uint_as_float((unsigned)h << 16));
+        }
+    }
+}
+
+// FP32 operands: tile rows ti*64.., tj*64.. of one simulation's Z
+__global__ __launch_bounds__(256) void k_gram_z32(const float* __restrict__ Z, int nw, int npp,
+                                                  float* __restrict__ fcd) {
+    __shared__ __align__(16) float As[ZT][ZLD32];
+    __shared__ __align__(16) float Bs[ZT][ZLD32];
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    int ti, tj;
+    tile_pair(blockIdx.x, ti, tj);
+    const int sim = blockIdx.z;
+    const float* Zs = Z + (size_t)sim * nw * npp;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wi = w >> 1, wj = w & 1;
+    v4 acc[2][2];
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) acc[a][b] = v4{0, 0, 0, 0};
+    // loader: 64 rows x 32 k = 512 float4 per operand, two per thread
+    const int lr = tid >> 3, lc = (tid & 7) * 4;   // rows lr and lr + 32, k lc..lc+3
+    auto ld = [&](int t, int r, int k0) -> float4 {
+        const int row = t * ZT + r;
+        return row < nw ? *reinterpret_cast<const float4*>(Zs + (size_t)row * npp + k0 + lc)
+                        : float4{0, 0, 0, 0};
+    };
+    float4 a0 = ld(ti, lr, 0), a1 = ld(ti, lr + 32, 0), b0 = ld(tj, lr, 0), b1 = ld(tj, lr + 32, 0);
+    for (int k0 = 0; k0 < npp; k0 += ZK32) {
+        *reinterpret_cast<float4*>(&As[lr][lc]) = a0;
+        *reinterpret_cast<float4*>(&As[lr + 32][lc]) = a1;
+        *reinterpret_cast<float4*>(&Bs[lr][lc]) = b0;
+        *reinterpret_cast<float4*>(&Bs[lr + 32][lc]) = b1;
+        __syncthreads();
+        if (k0 + ZK32 < npp) {   // next chunk in flight behind the MFMAs
+            a0 = ld(ti, lr, k0 + ZK32); a1 = ld(ti, lr + 32, k0 + ZK32);
+            b0 = ld(tj, lr, k0 + ZK32); b1 = ld(tj, lr + 32, k0 + ZK32);
+        }
+#pragma unroll
+        for (int s = 0; s < ZK32 / 4; ++s) {
+            float a[2], b[2];
+            for (int q = 0; q < 2; ++q) {
+                a[q] = As[wi * 32 + q * 16 + (lane & 15)][4 * s + (lane >> 4)];
+                b[q] = Bs[wj * 32 + q * 16 + (lane & 15)][4 * s + (lane >> 4)];
+            }
+            for (int qa = 0; qa < 2; ++qa)
+                for (int qb = 0; qb < 2; ++qb)
+                    acc[qa][qb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[qa], b[qb], acc[qa][qb], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    float* out = fcd + (size_t)sim * ((long long)nw * (nw - 1) / 2);
+    for (int qa = 0; qa < 2; ++qa)
+        for (int qb = 0; qb < 2; ++qb)
+            for (int r = 0; r < 4; ++r) {
+                const int row = ti * ZT + wi * 32 + qa * 16 + 4 * (lane >> 4) + r;
+                const int col = tj * ZT + wj * 32 + qb * 16 + (lane & 15);
+                if (row < nw && col < row) out[(long long)row * (row - 1) / 2 + col] = acc[qa][qb][r];
+            }
+}
+
+// BF16 operands (SPLIT: hi and lo arrays, three products)
+template <bool SPLIT>
+__global__ __launch_bounds__(256) void k_gram_zbf16(const unsigned short* __restrict__ Zh,
+                                                    const unsigned short* __restrict__ Zl, int nw, int npp,
+                                                    float* __restrict__ fcd) {
+    constexpr int NB = SPLIT ? 2 : 1;
+    __shared__ __align__(16) unsigned short As[NB][ZT][ZLDB];
+    __shared__ __align__(16) unsigned short Bs[NB][ZT][ZLDB];
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    int ti, tj;
+    tile_pair(blockIdx.x, ti, tj);
+    const int sim = blockIdx.z;
+    const size_t base = (size_t)sim * nw * npp;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wi = w >> 1, wj = w & 1;
+    v4 acc[2][2];
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) acc[a][b] = v4{0, 0, 0, 0};
+    // loader: 64 rows x 64 k of u16 = 512 x 16 B per operand and array, two per thread
+    const int lr = tid >> 3, lc = (tid & 7) * 8;
+    auto ld = [&](const unsigned short* Zp, int t, int r, int k0) -> u16x8 {
+        const int row = t * ZT + r;
+        return row < nw ? *reinterpret_cast<const u16x8*>(Zp + base + (size_t)row * npp + k0 + lc)
+                        : u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+    };
+    u16x8 ra[NB][2], rb[NB][2];
+    const unsigned short* Zp[2] = {Zh, Zl};
+    for (int h = 0; h < NB; ++h)
+        for (int q = 0; q < 2; ++q) { ra[h][q] = ld(Zp[h], ti, lr + 32 * q, 0); rb[h][q] = ld(Zp[h], tj, lr + 32 * q, 0); }
+    for (int k0 = 0; k0 < npp; k0 += ZKB) {
+        for (int h = 0; h < NB; ++h)
+            for (int q = 0; q < 2; ++q) {
+                *reinterpret_cast<u16x8*>(&As[h][lr + 32 * q][lc]) = ra[h][q];
+                *reinterpret_cast<u16x8*>(&Bs[h][lr + 32 * q][lc]) = rb[h][q];
+            }
+        __syncthreads();
+        if (k0 + ZKB < npp)
+            for (int h = 0; h < NB; ++h)
+                for (int q = 0; q < 2; ++q) {
+                    ra[h][q] = ld(Zp[h], ti, lr + 32 * q, k0 + ZKB);
+                    rb[h][q] = ld(Zp[h], tj, lr + 32 * q, k0 + ZKB);
+                }
+#pragma unroll
+        for (int s = 0; s < ZKB / 32; ++s) {
+            u16x8 a[NB][2], b[NB][2];
+            for (int h = 0; h < NB; ++h)
+                for (int q = 0; q < 2; ++q) {
+                    a[h][q] = *reinterpret_cast<const u16x8*>(&As[h][wi * 32 + q * 16 + (lane & 15)][32 * s + 8 * (lane >> 4)]);
+                    b[h][q] = *reinterpret_cast<const u16x8*>(&Bs[h][wj * 32 + q * 16 + (lane & 15)][32 * s + 8 * (lane >> 4)]);
+                }
+            for (int qa = 0; qa < 2; ++qa)
+                for (int qb = 0; qb < 2; ++qb) {
+                    acc[qa][qb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        __builtin_bit_cast(bf16x8, a[0][qa]), __builtin_bit_cast(bf16x8, b[0][qb]), acc[qa][qb], 0, 0, 0);
+                    if constexpr (SPLIT) {
+                        acc[qa][qb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            __builtin_bit_cast(bf16x8, a[0][qa]), __builtin_bit_cast(bf16x8, b[1][qb]), acc[qa][qb], 0, 0, 0);
+                        acc[qa][qb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            __builtin_bit_cast(bf16x8, a[1][qa]), __builtin_bit_cast(bf16x8, b[0][qb]), acc[qa][qb], 0, 0, 0);
+                    }
+                }
+        }
+        __syncthreads();
+    }
+    float* out = fcd + (size_t)sim * ((long long)nw * (nw - 1) / 2);
+    for (int qa = 0; qa < 2; ++qa)
+        for (int qb = 0; qb < 2; ++qb)
+            for (int r = 0; r < 4; ++r) {
+                const int row = ti * ZT + wi * 32 + qa * 16 + 4 * (lane >> 4) + r;
+                const int col = tj * ZT + wj * 32 + qb * 16 + (lane & 15);
+                if (row < nw && col < row) out[(long long)row * (row - 1) / 2 + col] = acc[qa][qb][r];
+            }
+}
+
+hipError_t launch_fcd_z(const float* wfc, int B, int nw, int np, const double* mean, const double* inv_ssd,
+                        void* zhi, unsigned short* zlo, int mode, float* fcd, hipStream_t st) {
+    const int npp = (np + 63) / 64 * 64;
+    const unsigned rows = (unsigned)((long long)B * nw);
+    if (mode == 0)
+        hipLaunchKernelGGL(k_zscore_rows<0>, dim3(rows), dim3(256), 0, st, wfc, np, npp, mean, inv_ssd, zhi, zlo);
+    else if (mode == 1)
+        hipLaunchKernelGGL(k_zscore_rows<1>, dim3(rows), dim3(256), 0, st, wfc, np, npp, mean, inv_ssd, zhi, zlo);
+    else
+        hipLaunchKernelGGL(k_zscore_rows<2>, dim3(rows), dim3(256), 0, st, wfc, np, npp, mean, inv_ssd, zhi, zlo);
+    const int nt = (nw + ZT - 1) / ZT;
+    const dim3 grid(nt * (nt + 1) / 2, 1, B);
+    if (mode == 0)
+        hipLaunchKernelGGL(k_gram_z32, grid, dim3(256), 0, st, (const float*)zhi, nw, npp, fcd);
+    else if (mode == 1)
+        hipLaunchKernelGGL(k_gram_zbf16<false>, grid, dim3(256), 0, st, (const unsigned short*)zhi, zlo, nw, npp,
+                           fcd);
+    else
+        hipLaunchKernelGGL(k_gram_zbf16<true>, grid, dim3(256), 0, st, (const unsigned short*)zhi, zlo, nw, npp,
+                           fcd);
+    return hipGetLastError();
+}
+
 // ---- launchers ---------------------------------------------------------------
 template <class T>
 hipError_t launch_seg_moments(const T* bold, int B, int N, int output_ts, int n_seg,

@@ -11,11 +11,13 @@ uneven_batch  four populations of different node counts and durations (N =
 fcd_stress    N = 400, 16 simulations, 1201 volumes at TR 0.72 s, FCD window 44
               step 1 (1157 windows x 79,800 edges): the BOLD of one FP32
               integration is analysed with FP64, FP32, BF16 and BF16x3 Gram
-              operands; times of the FC / FCD phases and KS / FC-r errors against
-              the FP64 Gram.
+              operands (and FP32 / BF16x3 with the FCD Gram z-scoring in its
+              tile loader instead of a separate pass); times of the FC / FCD
+              phases and KS / FC-r errors against the FP64 Gram.
 """
 from __future__ import annotations
 
+import os
 import sys
 import time
 
@@ -169,7 +171,12 @@ def fcd_stress(B=16, N=400, T=864_000, TR=720, ws=44, step=1):
     flop_fc = 2.0 * N * N * (ref["bold"].shape[1] + (ws // 2 * 2 + 1) * nw) * B
     flop_fcd = 2.0 * nw * nw * np_ * B
     rows = {}
-    for name, fcp in (("f64", _lib.FC_F64), ("f32", _lib.FC_F32), ("bf16", _lib.FC_BF16), ("bf16x3", _lib.FC_BF16X3)):
+    variants = [("f64", _lib.FC_F64, False), ("f32", _lib.FC_F32, False), ("bf16", _lib.FC_BF16, False),
+                ("bf16x3", _lib.FC_BF16X3, False), ("f32_direct", _lib.FC_F32, True),
+                ("bf16x3_direct", _lib.FC_BF16X3, True)]
+    for name, fcp, direct in variants:
+        # direct: the FCD Gram z-scores inside its tile loader (read at context creation)
+        os.environ["BNM_FCD_DIRECT"] = "1" if direct else "0"
         with _lib.Context(_lib.default_config(**base, fc_precision=fcp)) as c:
             c.set_empirical(*emp)
             c.set_bold(ref["bold"])
@@ -177,6 +184,7 @@ def fcd_stress(B=16, N=400, T=864_000, TR=720, ws=44, step=1):
             c.analyze()
             t = c.timings()
             o = c.outputs(bold=False)
+        os.environ.pop("BNM_FCD_DIRECT")
         ks_err = np.abs((o["corr"] - (1 - _pearson_rows(o["fc"], emp[0])) - np.abs(emp[0].mean() - o["fc"].mean(1)))
                         - (ref["corr"] - (1 - r_ref) - np.abs(emp[0].mean() - ref["fc"].mean(1))))
         rows[name] = {"fc_ms": round(t["fc"], 3), "fcd_ms": round(t["fcd"], 3), "fit_ms": round(t["fit"], 3),

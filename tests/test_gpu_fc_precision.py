@@ -59,3 +59,31 @@ def test_gram_precisions_against_fp64(oracle_mod, N, T, TR, ws, step):
     fcd_ref = oracle_mod.fcd_tril(o64["bold"][0], step, ws, False)
     np.testing.assert_allclose(o64["fc"][0], fc_ref, atol=1e-11)
     np.testing.assert_allclose(o64["fcd"][0], fcd_ref, atol=1e-10)
+
+
+@pytest.mark.parametrize("fcp", [bnm.FC_F32, bnm.FC_BF16, bnm.FC_BF16X3])
+def test_fcd_zscored_pass_matches_loader_zscoring(monkeypatch, fcp):
+    """The FCD Gram over rows z-scored in a separate pass (default) against the
+    Gram that z-scores inside its tile loader (BNM_FCD_DIRECT=1): the same
+    operands up to FP32 rounding of the z-scores, so FCD agrees to ~1e-5.
+    nw = 101 windows spans two 64-row tiles (a diagonal and an off-diagonal
+    one, with a ragged edge)."""
+    N, T, TR, ws, step, B = 68, 130_000, 1000, 30, 1, 3
+    print("case: fcd z-scored pass fc_precision", fcp, flush=True)
+    SC = W.synthetic_sc(N, 0.3, 9)
+    g, ee, ei = W.population(B, 10)
+    params = (g, W.expand_homogeneous(ee, N), W.expand_homogeneous(ei, N))
+    noise = bnm.make_noise_chunked(11, (T + 1) * 20 * N)
+    np_ = bnm.n_pairs(N, False)
+    rng = np.random.default_rng(12)
+    emp = (rng.uniform(-0.2, 0.8, np_), rng.uniform(-0.5, 1.0, 3000))
+    monkeypatch.setenv("BNM_FCD_DIRECT", "0")
+    oz = _run(fcp, N, B, T, TR, ws, step, SC, params, noise, emp)
+    monkeypatch.setenv("BNM_FCD_DIRECT", "1")
+    od = _run(fcp, N, B, T, TR, ws, step, SC, params, noise, emp)
+    assert oz["fcd"].shape[1] == 101 * 100 // 2
+    np.testing.assert_array_equal(oz["fc"], od["fc"])
+    d = np.abs(oz["fcd"] - od["fcd"]).max()
+    print(f"fcd max |z-pass - loader| {d:.2e}")
+    assert d <= 2e-5
+    assert np.abs(oz["corr"] - od["corr"]).max() <= 1e-3
