@@ -60,7 +60,8 @@ int dalloc(T** p, size_t n) {
 // One launchable integration kernel shape.
 struct Variant {
     int kind;   // 0 split (SC in registers), 1 stream (SC from L2), 2 mfma, 3 plain,
-                // 4 split with the E/I pools on separate lanes (FP32),
+                // 4 split with the E/I pools on separate lanes (FP32; mode 1:
+                //   S_E by DPP row broadcast),
                 // 5 stream with register/LDS-resident SC (FP32)
     int ls, ks, mode;
 };
@@ -114,7 +115,7 @@ struct bnm_ctx {
     Variant variant{0, 4, 1, 0};
     hipGraphExec_t graph_exec = nullptr;
     bool have_sc = false, have_noise = false, have_emp = false, have_params = false;
-    int fast_ls = 0, fast_ks = 0, stream_ks = 0, force_stream = 0, no_ei = 0, ei_ls = 0, no_hyb = 0, hyb_ks = 0;
+    int fast_ls = 0, fast_ks = 0, stream_ks = 0, force_stream = 0, no_ei = 0, ei_ls = 0, ei_bc = 0, no_hyb = 0, hyb_ks = 0;
 };
 
 extern "C" {
@@ -454,6 +455,10 @@ bnm_ctx* bnm_ctx_create(const bnm_config* cfg, int* status) {
     x->force_stream = env_int("BNM_FORCE_STREAM", 0);
     x->no_ei = env_int("BNM_NO_EI", 0);
     x->ei_ls = env_int("BNM_EI_LS", 0);
+    // BNM_EI_BC=1 adds the DPP-broadcast E/I variant to tuning (or pins it
+    // with BNM_EI_LS=4); off by default: bit-identical to the LDS form and
+    // slower at N = 100 (0.447 vs 0.382 us per substep)
+    x->ei_bc = env_int("BNM_EI_BC", 0);
     x->no_hyb = env_int("BNM_NO_HYB", 0);
     x->hyb_ks = env_int("BNM_HYB_KS", 0);
     {
@@ -586,7 +591,7 @@ static bool variant_ok(const bnm_ctx* x, const Variant& v) {
         case 0: return fast_shape(x->N, v.ls, v.ks, el).ok;
         case 1: return stream_shape(x->N, v.ks, el).ok;
         case 2: return !x->f64 && mfma_shape(x->N, v.mode).ok;
-        case 4: return !x->f64 && ei_shape(x->N, v.ls).ok;
+        case 4: return !x->f64 && ei_shape(x->N, v.ls, v.mode == 1).ok;
         case 5: return !x->f64 && hyb_shape(x->N, v.ks, x->lds_max).ok;
         default: return true;
     }
@@ -614,9 +619,9 @@ static std::string variant_name(const bnm_ctx* x, const Variant& v) {
         std::snprintf(name, sizeof name, "hyb_f32(KS=%d,NW=%d,kc=%d,regs=%d,lds=%d,streamed=%d,threads=%d)", g.KS,
                       g.NW, g.kc, g.KC, g.SS, g.kc - g.KC - g.SS, g.threads);
     } else if (v.kind == 4) {
-        const FastShape s = ei_shape(x->N, v.ls);
-        std::snprintf(name, sizeof name, "split_ei_f32(LS=%d,NW=%d,KC=%d,SS=%d,threads=%d)", s.LS, s.NW, s.KC,
-                      s.SS, s.threads);
+        const FastShape s = ei_shape(x->N, v.ls, v.mode == 1);
+        std::snprintf(name, sizeof name, "split_ei_f32(LS=%d,NW=%d,KC=%d,SS=%d,threads=%d%s)", s.LS, s.NW, s.KC,
+                      s.SS, s.threads, v.mode == 1 ? ",dpp" : "");
     } else {
         std::snprintf(name, sizeof name, "plain_%s(threads=%d)", p, ((x->N + 63) / 64) * 64);
     }
@@ -632,7 +637,7 @@ static hipError_t launch_variant(bnm_ctx* x, const Variant& v, const SimArgs& a,
             return x->f64 ? launch_sim_stream<double>(x->SCt64, a, v.ks, st)
                           : launch_sim_stream<float>(x->SCt32, a, v.ks, st);
         case 2: return launch_sim_mfma(x->SC32, a, v.mode, st);
-        case 4: return launch_sim_ei(x->SC32, a, v.ls, st);
+        case 4: return launch_sim_ei(x->SC32, a, v.ls, v.mode == 1, st);
         case 5: return launch_sim_hyb(x->SCc32, a, v.ks, x->lds_max, st);
         default:
             return x->f64 ? launch_sim_plain<double>(x->SC64, a, st)
@@ -663,7 +668,7 @@ static std::vector<Variant> candidates(const bnm_ctx* x) {
         if (!out.empty()) return out;
     }
     if (x->ei_ls > 0 && !x->f64) {
-        add({4, x->ei_ls, 1, 0});
+        add({4, x->ei_ls, 1, x->ei_bc == 1 ? 1 : 0});
         if (!out.empty()) return out;
     }
     const bool pinned = x->fast_ls > 0 || x->fast_ks > 0;
@@ -673,6 +678,7 @@ static std::vector<Variant> candidates(const bnm_ctx* x) {
     }
     if (!x->force_stream) {
         if (!x->f64 && !x->no_ei) {
+            if (x->ei_bc == 1) add({4, 4, 1, 1});
             add({4, 4, 1, 0});
             add({4, 2, 1, 0});
         }

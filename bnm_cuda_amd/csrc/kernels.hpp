@@ -36,8 +36,8 @@ FastShape hyb_shape(int N, int KS, size_t lds_bytes);
 // SCc: chunked layout SCc[((k/16)*N + j)*16 + k%16] = SC[j][k], zero for k >= N
 hipError_t launch_sim_hyb(const float* SCc, const SimArgs& a, int KS, size_t lds_bytes, hipStream_t st);
 // FP32 split kernel with the node update divided between lanes by population
-FastShape ei_shape(int N, int LS);
-hipError_t launch_sim_ei(const float* SC, const SimArgs& a, int LS, hipStream_t st);
+FastShape ei_shape(int N, int LS, bool bc = false);
+hipError_t launch_sim_ei(const float* SC, const SimArgs& a, int LS, bool bc, hipStream_t st);
 template <class R>
 hipError_t launch_sim_stream(const R* SCt, const SimArgs& a, int KS, hipStream_t st);
 

@@ -332,11 +332,51 @@ __global__ __launch_bounds__(MAXT) void k_sim_split(const R* __restrict__ SC, Si
 // one pool's update instead of both, which is what bounds the split kernel
 // (two waves per SIMD at N = 100).  LS = 2 or 4, KS = 1; slices 2 and 3 of
 // LS = 4 mirror slices 0 and 1.
+//
+// BC (LS = 4, KC <= 32): S_E reaches the coupling dot by DPP row broadcast
+// instead of LDS vector reads.  With LS = 4 a slice is one 16-lane DPP row;
+// lane q of the row reads only S_E[slice base + q] (and + 16) from LDS, and
+// v_fmac_f32_dpp row_newbcast:q multiplies lane q's value into every lane's
+// accumulator.  Every lane of a slice otherwise reads the whole slice, so
+// the LDS returns N^2 * 4 bytes per substep (40 KB at N = 100) whatever the
+// slicing; the broadcast form returns 2 * 4 bytes per lane.
 // ---------------------------------------------------------------------------
-template <int KC, int LS, int MAXT>
+
+// acc[i] += (lane 4G + i of this DPP row's src) * sc[i], i = 0..3.  The
+// leading s_nop 1 provides the two wait states a DPP read needs after a VALU
+// write of its source register.
+template <int G>
+__device__ __forceinline__ void bcast_fma4(float (&acc)[4], float src, float s0, float s1, float s2, float s3) {
+#define BNM_BC4(Q0, Q1, Q2, Q3)                                                                  \
+    asm("s_nop 1\n\t"                                                                            \
+        "v_fmac_f32_dpp %0, %4, %5 row_newbcast:" #Q0 " row_mask:0xf bank_mask:0xf\n\t"           \
+        "v_fmac_f32_dpp %1, %4, %6 row_newbcast:" #Q1 " row_mask:0xf bank_mask:0xf\n\t"           \
+        "v_fmac_f32_dpp %2, %4, %7 row_newbcast:" #Q2 " row_mask:0xf bank_mask:0xf\n\t"           \
+        "v_fmac_f32_dpp %3, %4, %8 row_newbcast:" #Q3 " row_mask:0xf bank_mask:0xf"                 \
+        : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]), "+v"(acc[3])                                  \
+        : "v"(src), "v"(s0), "v"(s1), "v"(s2), "v"(s3))
+    if constexpr (G == 0) BNM_BC4(0, 1, 2, 3);
+    else if constexpr (G == 1) BNM_BC4(4, 5, 6, 7);
+    else if constexpr (G == 2) BNM_BC4(8, 9, 10, 11);
+    else BNM_BC4(12, 13, 14, 15);
+#undef BNM_BC4
+}
+
+// sum over q < NQ of sc[Q0 + q] * (lane q of src's row), into 4 accumulators
+template <int NQ, int Q0, int KC>
+__device__ __forceinline__ void bcast_dot(float (&acc)[4], float src, const float (&sc)[KC]) {
+    static_assert(NQ % 4 == 0 && NQ <= 16, "groups of 4 lanes of one row");
+    if constexpr (NQ >= 4) bcast_fma4<0>(acc, src, sc[Q0], sc[Q0 + 1], sc[Q0 + 2], sc[Q0 + 3]);
+    if constexpr (NQ >= 8) bcast_fma4<1>(acc, src, sc[Q0 + 4], sc[Q0 + 5], sc[Q0 + 6], sc[Q0 + 7]);
+    if constexpr (NQ >= 12) bcast_fma4<2>(acc, src, sc[Q0 + 8], sc[Q0 + 9], sc[Q0 + 10], sc[Q0 + 11]);
+    if constexpr (NQ >= 16) bcast_fma4<3>(acc, src, sc[Q0 + 12], sc[Q0 + 13], sc[Q0 + 14], sc[Q0 + 15]);
+}
+
+template <int KC, int LS, int MAXT, bool BC>
 __global__ __launch_bounds__(MAXT) void k_sim_ei(const float* __restrict__ SC, SimArgs a, int NW, int SS) {
     static_assert(LS == 2 || LS == 4, "E/I split needs two lanes per node");
     static_assert(KC % 4 == 0, "slice length must be a multiple of 4");
+    static_assert(!BC || (LS == 4 && KC <= 32), "row broadcast: 16-lane slices of at most 32 values");
     constexpr int GN = 64 / LS;
     const int sim = blockIdx.x;
     const int N = a.N;
@@ -404,9 +444,15 @@ __global__ __launch_bounds__(MAXT) void k_sim_ei(const float* __restrict__ SC, S
 #pragma unroll
         for (int s = 0; s < 10; ++s) {
             const float4* Sr = reinterpret_cast<const float4*>(Sb + buf * ldS + sl * SS);
-            float4 sv[KC / 4];
+            float4 sv[BC ? 1 : KC / 4];
+            float v0 = 0.0f, v1 = 0.0f;
+            if constexpr (BC) {
+                v0 = Sb[buf * ldS + sl * SS + ln];
+                if constexpr (KC > 16) v1 = Sb[buf * ldS + sl * SS + 16 + ln];
+            } else {
 #pragma unroll
-            for (int q = 0; q < KC / 4; ++q) sv[q] = Sr[q];
+                for (int q = 0; q < KC / 4; ++q) sv[q] = Sr[q];
+            }
             // previous-substep S of the partner pool, and the part of I that
             // does not wait for the coupling
             float So;
@@ -419,12 +465,17 @@ __global__ __launch_bounds__(MAXT) void k_sim_ei(const float* __restrict__ SC, S
             const float SEv = pop ? So : S, SIv = pop ? S : So;
             const float pre = k0 + kSE * SEv - kSI * SIv;
             float acc[4] = {0, 0, 0, 0};
+            if constexpr (BC) {
+                bcast_dot<(KC < 16 ? KC : 16), 0>(acc, v0, sc);
+                if constexpr (KC > 16) bcast_dot<KC - 16, 16>(acc, v1, sc);
+            } else {
 #pragma unroll
-            for (int q = 0; q < KC / 4; ++q) {
-                acc[0] = fmaf(sc[4 * q + 0], sv[q].x, acc[0]);
-                acc[1] = fmaf(sc[4 * q + 1], sv[q].y, acc[1]);
-                acc[2] = fmaf(sc[4 * q + 2], sv[q].z, acc[2]);
-                acc[3] = fmaf(sc[4 * q + 3], sv[q].w, acc[3]);
+                for (int q = 0; q < KC / 4; ++q) {
+                    acc[0] = fmaf(sc[4 * q + 0], sv[q].x, acc[0]);
+                    acc[1] = fmaf(sc[4 * q + 1], sv[q].y, acc[1]);
+                    acc[2] = fmaf(sc[4 * q + 2], sv[q].z, acc[2]);
+                    acc[3] = fmaf(sc[4 * q + 3], sv[q].w, acc[3]);
+                }
             }
             float c = (acc[0] + acc[1]) + (acc[2] + acc[3]);
             if constexpr (LS == 4) c = lane_sum16<float>(c);
@@ -856,36 +907,48 @@ hipError_t launch_sim_fast(const R* SC, const SimArgs& a, int LS, int KS, hipStr
     return hipGetLastError();
 }
 
-template <int KC, int LS>
+template <int KC, int LS, bool BC>
 static void launch_ei(const float* SC, const SimArgs& a, const FastShape& s, hipStream_t st) {
     constexpr int MAXT = maxt_for(KC, 4);
-    hipLaunchKernelGGL((k_sim_ei<KC, LS, MAXT>), dim3(a.B), dim3(s.threads), s.smem, st, SC, a, s.NW, s.SS);
+    hipLaunchKernelGGL((k_sim_ei<KC, LS, MAXT, BC>), dim3(a.B), dim3(s.threads), s.smem, st, SC, a, s.NW, s.SS);
 }
 
 template <int KC>
-static void launch_ei_kc(const float* SC, const SimArgs& a, const FastShape& s, hipStream_t st) {
-    if (s.LS == 2) launch_ei<KC, 2>(SC, a, s, st);
-    else launch_ei<KC, 4>(SC, a, s, st);
+static void launch_ei_kc(const float* SC, const SimArgs& a, const FastShape& s, bool bc, hipStream_t st) {
+    if constexpr (KC <= 32) {
+        if (bc) { launch_ei<KC, 4, true>(SC, a, s, st); return; }
+    }
+    if (s.LS == 2) launch_ei<KC, 2, false>(SC, a, s, st);
+    else launch_ei<KC, 4, false>(SC, a, s, st);
 }
 
-FastShape ei_shape(int N, int LS) {
+FastShape ei_shape(int N, int LS, bool bc) {
     FastShape s = fast_shape(N, LS, 1, sizeof(float));
     s.ok = s.ok && (LS == 2 || LS == 4);
+    if (bc) {
+        // lane q of slice x reads slots x*SS + q and x*SS + 16 + q: SS = 16
+        // keeps one-value slices contiguous; two-value slices take SS = 48
+        // (in bounds, and the two slices of a half-wave on disjoint banks)
+        s.ok = s.ok && LS == 4 && s.KC <= 32;
+        s.SS = s.KC <= 16 ? 16 : 48;
+        s.smem = (2 * (size_t)LS * s.SS + (size_t)s.NW * (64 / LS)) * sizeof(float);
+        s.ok = s.ok && s.smem <= 64 * 1024;
+    }
     return s;
 }
 
-hipError_t launch_sim_ei(const float* SC, const SimArgs& a, int LS, hipStream_t st) {
-    const FastShape s = ei_shape(a.N, LS);
+hipError_t launch_sim_ei(const float* SC, const SimArgs& a, int LS, bool bc, hipStream_t st) {
+    const FastShape s = ei_shape(a.N, LS, bc);
     if (!s.ok) return hipErrorInvalidValue;
     switch (s.KC) {
-        case 8: launch_ei_kc<8>(SC, a, s, st); break;
-        case 16: launch_ei_kc<16>(SC, a, s, st); break;
-        case 28: launch_ei_kc<28>(SC, a, s, st); break;
-        case 32: launch_ei_kc<32>(SC, a, s, st); break;
-        case 52: launch_ei_kc<52>(SC, a, s, st); break;
-        case 64: launch_ei_kc<64>(SC, a, s, st); break;
-        case 100: launch_ei_kc<100>(SC, a, s, st); break;
-        default: launch_ei_kc<128>(SC, a, s, st); break;
+        case 8: launch_ei_kc<8>(SC, a, s, bc, st); break;
+        case 16: launch_ei_kc<16>(SC, a, s, bc, st); break;
+        case 28: launch_ei_kc<28>(SC, a, s, bc, st); break;
+        case 32: launch_ei_kc<32>(SC, a, s, bc, st); break;
+        case 52: launch_ei_kc<52>(SC, a, s, bc, st); break;
+        case 64: launch_ei_kc<64>(SC, a, s, bc, st); break;
+        case 100: launch_ei_kc<100>(SC, a, s, bc, st); break;
+        default: launch_ei_kc<128>(SC, a, s, bc, st); break;
     }
     return hipGetLastError();
 }

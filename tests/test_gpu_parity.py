@@ -202,6 +202,44 @@ def test_f32_ei_split_kernel_within_bounds(oracle_mod, case, ref, ls, monkeypatc
     assert np.all(d_ext <= np.array([1e-3, 1e-3, 0.05, 1e-3, 1e-3, 0.05]))
 
 
+def _ei_run(n, T, bc, monkeypatch, seed=0):
+    monkeypatch.setenv("BNM_EI_LS", "4")
+    monkeypatch.setenv("BNM_EI_BC", "1" if bc else "0")
+    Bn = 8
+    SC = W.synthetic_sc(n, 0.3, 30 + seed)
+    g, ee, ei = W.population(Bn, 31 + seed)
+    cfg = bnm.default_config(nodes=n, n_sims=Bn, time_steps=T, bold_tr=1000, n_vols_remove=0, window_size=10,
+                             window_step=2, noise_period_ms=T + 1, precision=bnm.PREC_F32,
+                             kernel=bnm.KERNEL_FAST, fc_precision=bnm.FC_F32, max_fic_trials=3)
+    rng = np.random.default_rng(32)
+    with bnm.Context(cfg) as c:
+        c.set_sc(SC)
+        c.set_noise(bnm.make_noise_chunked(33 + seed, (T + 1) * 20 * n))
+        c.set_empirical(rng.uniform(-0.2, 0.8, bnm.n_pairs(n, True)), rng.uniform(-0.5, 1.0, 100))
+        c.set_params(g, W.expand_homogeneous(ee, n), W.expand_homogeneous(ei, n))
+        c.run()
+        return c.kernel_name(), c.outputs()
+
+
+@pytest.mark.parametrize("n", [20, 64, 100, 128])
+def test_f32_ei_dpp_broadcast_bit_identical(n, monkeypatch):
+    """The E/I kernel that spreads S_E over each 16-lane slice by DPP row
+    broadcast (v_fmac_f32_dpp row_newbcast) against the same kernel reading
+    the slice from LDS: the products are accumulated in the same order, so
+    every output is bit-identical.  n = 20 / 64 use one broadcast source
+    register per lane (slices of 8 / 16), n = 100 / 128 two (28 / 32)."""
+    print("case: ei dpp broadcast N", n, flush=True)
+    T = 30_000
+    kb, b = _ei_run(n, T, True, monkeypatch)
+    kl, l = _ei_run(n, T, False, monkeypatch)
+    print(kb, kl)
+    assert kb.startswith("split_ei_f32(LS=4") and kb.endswith(",dpp)")
+    assert kl.startswith("split_ei_f32(LS=4") and not kl.endswith(",dpp)")
+    for k in ("bold", "ext", "wIE_fic", "fic_trials", "fic_failed", "fc", "fcd", "corr"):
+        np.testing.assert_array_equal(b[k], l[k], err_msg=k)
+    assert np.isfinite(b["bold"]).all() and np.abs(b["bold"]).max() > 0
+
+
 @pytest.mark.parametrize("fcp", [bnm.FC_F64, bnm.FC_F32])
 def test_analyze_given_bold_matches_oracle(case, ref, fcp):
     """bnm_ctx_set_bold + bnm_ctx_analyze on the oracle's BOLD reproduce the
