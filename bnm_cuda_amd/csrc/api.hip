@@ -60,8 +60,9 @@ int dalloc(T** p, size_t n) {
 // One launchable integration kernel shape.
 struct Variant {
     int kind;   // 0 split (SC in registers), 1 stream (SC from L2), 2 mfma, 3 plain,
-                // 4 split with the E/I pools on separate lanes (FP32; mode 1:
-                //   S_E by DPP row broadcast),
+                // 4 split with the E/I pools on separate lanes (FP32; mode = form:
+                //   0 as written, 1 S_E by DPP row broadcast, 2 / 3 short-chain
+                //   update with four / one partial sums),
                 // 5 stream with register/LDS-resident SC (FP32)
     int ls, ks, mode;
 };
@@ -115,7 +116,7 @@ struct bnm_ctx {
     Variant variant{0, 4, 1, 0};
     hipGraphExec_t graph_exec = nullptr;
     bool have_sc = false, have_noise = false, have_emp = false, have_params = false;
-    int fast_ls = 0, fast_ks = 0, stream_ks = 0, force_stream = 0, no_ei = 0, ei_ls = 0, ei_bc = 0, no_hyb = 0, hyb_ks = 0;
+    int fast_ls = 0, fast_ks = 0, stream_ks = 0, force_stream = 0, no_ei = 0, ei_ls = 0, ei_bc = 0, ei_form = -1, no_hyb = 0, hyb_ks = 0;
 };
 
 extern "C" {
@@ -459,6 +460,8 @@ bnm_ctx* bnm_ctx_create(const bnm_config* cfg, int* status) {
     // with BNM_EI_LS=4); off by default: bit-identical to the LDS form and
     // slower at N = 100 (0.447 vs 0.382 us per substep)
     x->ei_bc = env_int("BNM_EI_BC", 0);
+    // BNM_EI_FORM pins the E/I update form (0 as written, 2 / 3 short chain)
+    x->ei_form = env_int("BNM_EI_FORM", -1);
     x->no_hyb = env_int("BNM_NO_HYB", 0);
     x->hyb_ks = env_int("BNM_HYB_KS", 0);
     {
@@ -620,8 +623,9 @@ static std::string variant_name(const bnm_ctx* x, const Variant& v) {
                       g.NW, g.kc, g.KC, g.SS, g.kc - g.KC - g.SS, g.threads);
     } else if (v.kind == 4) {
         const FastShape s = ei_shape(x->N, v.ls, v.mode == 1);
+        static const char* fm[] = {"", ",dpp", ",short4", ",short1"};
         std::snprintf(name, sizeof name, "split_ei_f32(LS=%d,NW=%d,KC=%d,SS=%d,threads=%d%s)", s.LS, s.NW, s.KC,
-                      s.SS, s.threads, v.mode == 1 ? ",dpp" : "");
+                      s.SS, s.threads, fm[v.mode & 3]);
     } else {
         std::snprintf(name, sizeof name, "plain_%s(threads=%d)", p, ((x->N + 63) / 64) * 64);
     }
@@ -637,7 +641,7 @@ static hipError_t launch_variant(bnm_ctx* x, const Variant& v, const SimArgs& a,
             return x->f64 ? launch_sim_stream<double>(x->SCt64, a, v.ks, st)
                           : launch_sim_stream<float>(x->SCt32, a, v.ks, st);
         case 2: return launch_sim_mfma(x->SC32, a, v.mode, st);
-        case 4: return launch_sim_ei(x->SC32, a, v.ls, v.mode == 1, st);
+        case 4: return launch_sim_ei(x->SC32, a, v.ls, v.mode, st);
         case 5: return launch_sim_hyb(x->SCc32, a, v.ks, x->lds_max, st);
         default:
             return x->f64 ? launch_sim_plain<double>(x->SC64, a, st)
@@ -668,7 +672,7 @@ static std::vector<Variant> candidates(const bnm_ctx* x) {
         if (!out.empty()) return out;
     }
     if (x->ei_ls > 0 && !x->f64) {
-        add({4, x->ei_ls, 1, x->ei_bc == 1 ? 1 : 0});
+        add({4, x->ei_ls, 1, x->ei_form >= 0 ? x->ei_form : x->ei_bc == 1 ? 1 : 0});
         if (!out.empty()) return out;
     }
     const bool pinned = x->fast_ls > 0 || x->fast_ks > 0;
@@ -679,8 +683,12 @@ static std::vector<Variant> candidates(const bnm_ctx* x) {
     if (!x->force_stream) {
         if (!x->f64 && !x->no_ei) {
             if (x->ei_bc == 1) add({4, 4, 1, 1});
-            add({4, 4, 1, 0});
-            add({4, 2, 1, 0});
+            // forms 3 and 2 (short chain) are tuned by default; form 0 only when pinned
+            for (int f : {3, 2, 0})
+                if (x->ei_form < 0 ? f != 0 : x->ei_form == f) {
+                    add({4, 4, 1, f});
+                    add({4, 2, 1, f});
+                }
         }
         static const int pref[][2] = {{4, 1}, {2, 1}, {1, 1}, {2, 2}, {1, 2}, {4, 2},
                                       {1, 4}, {2, 4}, {1, 8}, {2, 8}, {1, 16}};

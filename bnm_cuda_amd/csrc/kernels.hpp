@@ -37,7 +37,9 @@ FastShape hyb_shape(int N, int KS, size_t lds_bytes);
 hipError_t launch_sim_hyb(const float* SCc, const SimArgs& a, int KS, size_t lds_bytes, hipStream_t st);
 // FP32 split kernel with the node update divided between lanes by population
 FastShape ei_shape(int N, int LS, bool bc = false);
-hipError_t launch_sim_ei(const float* SC, const SimArgs& a, int LS, bool bc, hipStream_t st);
+// form: 0 as written, 1 DPP row broadcast (LS = 4, N <= 128), 2 / 3 short-chain
+// update with four / one partial sums in the coupling dot
+hipError_t launch_sim_ei(const float* SC, const SimArgs& a, int LS, int form, hipStream_t st);
 template <class R>
 hipError_t launch_sim_stream(const R* SCt, const SimArgs& a, int KS, hipStream_t st);
 

@@ -372,8 +372,18 @@ __device__ __forceinline__ void bcast_dot(float (&acc)[4], float src, const floa
     if constexpr (NQ >= 16) bcast_fma4<3>(acc, src, sc[Q0 + 12], sc[Q0 + 13], sc[Q0 + 14], sc[Q0 + 15]);
 }
 
-template <int KC, int LS, int MAXT, bool BC>
+// FORM: 0 the update as written above; 1 the same with the DPP row broadcast;
+// 2 / 3 the short-chain update: everything that does not depend on the
+// coupling c is formed before it arrives, so that after the reduction
+//   x = aa*kc*c + (aa*pre - bb),  y = ndl*aa*kc*c + ndl*(aa*pre - bb)
+//   r = rate(x, exp2(y)),         S = clamp(fma(r, g - gm*S, S + xi*nzs - it*S))
+// (eleven instructions instead of seventeen; the same expressions up to FP32
+// rounding), I_E is formed only on the last substep of a millisecond (its
+// only use), and S_E / S_I come straight from the two halves of the lane
+// swap.  FORM 2 keeps four partial sums in the dot, FORM 3 one chain.
+template <int KC, int LS, int MAXT, int FORM>
 __global__ __launch_bounds__(MAXT) void k_sim_ei(const float* __restrict__ SC, SimArgs a, int NW, int SS) {
+    constexpr bool BC = FORM == 1, LEAN = FORM >= 2;
     static_assert(LS == 2 || LS == 4, "E/I split needs two lanes per node");
     static_assert(KC % 4 == 0, "slice length must be a multiple of 4");
     static_assert(!BC || (LS == 4 && KC <= 32), "row broadcast: 16-lane slices of at most 32 values");
@@ -418,6 +428,12 @@ __global__ __launch_bounds__(MAXT) void k_sim_ei(const float* __restrict__ SC, S
     const float gm = pop ? 0.0f : g;
     const float it = pop ? (float)(C64::dt / C64::tau_I) : (float)(C64::dt / C64::tau_E);
     constexpr float nzs = (float)(C64::sigma * C64::sqrt_dt);
+    // short-chain form: products of the pool constants, formed in FP64
+    const double aa64 = pop ? C64::a_I : C64::a_E, bb64 = pop ? C64::b_I : C64::b_E;
+    const double ndl64 = pop ? -C64::d_I * kLog2e : -C64::d_E * kLog2e;
+    const double kc64 = pop ? 0.0 : (double)GJ;
+    const float aakc = (float)(aa64 * kc64), ndlaakc = (float)(ndl64 * aa64 * kc64);
+    const float ndlaa = (float)(ndl64 * aa64), ndlbb = (float)(ndl64 * bb64);
     float wIE = comp ? (float)a.wIE_in[o] : 0.0f, delta = (float)a.init_delta;
     float S = 0.001f, I = 0.0f, r = 0.0f;
     BW<float> bw;
@@ -461,6 +477,54 @@ __global__ __launch_bounds__(MAXT) void k_sim_ei(const float* __restrict__ SC, S
                                  : __builtin_amdgcn_permlane32_swap(__float_as_uint(S), __float_as_uint(S), false, false);
                 // element 0: lower-row lane's value, element 1: upper-row lane's
                 So = __uint_as_float(pop ? w[0] : w[1]);
+            }
+            if constexpr (LEAN) {
+                // S_E is the lower row's value of the swap pair, S_I the upper's
+                // (slices 2 and 3 mirror 0 and 1), in every lane
+                const auto w = LS == 4 ? __builtin_amdgcn_permlane16_swap(__float_as_uint(S), __float_as_uint(S), false, false)
+                                       : __builtin_amdgcn_permlane32_swap(__float_as_uint(S), __float_as_uint(S), false, false);
+                const float pre = fmaf(-kSI, __uint_as_float(w[1]), fmaf(kSE, __uint_as_float(w[0]), k0));
+                const float X0 = fmaf(aa, pre, -bb);
+                const float Y0 = fmaf(ndlaa, pre, -ndlbb);
+                const float Ac = fmaf(-gm, S, g);
+                const float SB = fmaf(-it, S, fmaf(nz[s], nzs, S));
+                float c;
+                if constexpr (FORM == 3) {
+                    c = sc[0] * sv[0].x;
+                    c = fmaf(sc[1], sv[0].y, c);
+                    c = fmaf(sc[2], sv[0].z, c);
+                    c = fmaf(sc[3], sv[0].w, c);
+#pragma unroll
+                    for (int q = 1; q < KC / 4; ++q) {
+                        c = fmaf(sc[4 * q + 0], sv[q].x, c);
+                        c = fmaf(sc[4 * q + 1], sv[q].y, c);
+                        c = fmaf(sc[4 * q + 2], sv[q].z, c);
+                        c = fmaf(sc[4 * q + 3], sv[q].w, c);
+                    }
+                } else {
+                    float acc[4] = {sc[0] * sv[0].x, sc[1] * sv[0].y, sc[2] * sv[0].z, sc[3] * sv[0].w};
+#pragma unroll
+                    for (int q = 1; q < KC / 4; ++q) {
+                        acc[0] = fmaf(sc[4 * q + 0], sv[q].x, acc[0]);
+                        acc[1] = fmaf(sc[4 * q + 1], sv[q].y, acc[1]);
+                        acc[2] = fmaf(sc[4 * q + 2], sv[q].z, acc[2]);
+                        acc[3] = fmaf(sc[4 * q + 3], sv[q].w, acc[3]);
+                    }
+                    c = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+                }
+                if constexpr (LS == 4) c = lane_sum16<float>(c);
+                c = lane_sum32<float>(c);
+                const float x = fmaf(aakc, c, X0);
+                const float e = __builtin_amdgcn_exp2f(fmaf(ndlaakc, c, Y0));
+                const float direct = x * __builtin_amdgcn_rcpf(1.0f - e);
+                const float series = invd + x * (0.5f + d12 * x);
+                r = fabsf(x) < 0.1f ? series : direct;
+                S = __builtin_amdgcn_fmed3f(fmaf(r, Ac, SB), 0.0f, 1.0f);
+                if (s == 9) I = fmaf(kc, c, pre);
+                if (writer) Sb[(buf ^ 1) * ldS + posj] = S;
+                __syncthreads();
+                buf ^= 1;
+                continue;
             }
             const float SEv = pop ? So : S, SIv = pop ? S : So;
             const float pre = k0 + kSE * SEv - kSI * SIv;
@@ -907,19 +971,26 @@ hipError_t launch_sim_fast(const R* SC, const SimArgs& a, int LS, int KS, hipStr
     return hipGetLastError();
 }
 
-template <int KC, int LS, bool BC>
+template <int KC, int LS, int FORM>
 static void launch_ei(const float* SC, const SimArgs& a, const FastShape& s, hipStream_t st) {
     constexpr int MAXT = maxt_for(KC, 4);
-    hipLaunchKernelGGL((k_sim_ei<KC, LS, MAXT, BC>), dim3(a.B), dim3(s.threads), s.smem, st, SC, a, s.NW, s.SS);
+    hipLaunchKernelGGL((k_sim_ei<KC, LS, MAXT, FORM>), dim3(a.B), dim3(s.threads), s.smem, st, SC, a, s.NW, s.SS);
+}
+
+template <int KC, int LS>
+static void launch_ei_form(const float* SC, const SimArgs& a, const FastShape& s, int form, hipStream_t st) {
+    if (form == 2) launch_ei<KC, LS, 2>(SC, a, s, st);
+    else if (form == 3) launch_ei<KC, LS, 3>(SC, a, s, st);
+    else launch_ei<KC, LS, 0>(SC, a, s, st);
 }
 
 template <int KC>
-static void launch_ei_kc(const float* SC, const SimArgs& a, const FastShape& s, bool bc, hipStream_t st) {
+static void launch_ei_kc(const float* SC, const SimArgs& a, const FastShape& s, int form, hipStream_t st) {
     if constexpr (KC <= 32) {
-        if (bc) { launch_ei<KC, 4, true>(SC, a, s, st); return; }
+        if (form == 1) { launch_ei<KC, 4, 1>(SC, a, s, st); return; }
     }
-    if (s.LS == 2) launch_ei<KC, 2, false>(SC, a, s, st);
-    else launch_ei<KC, 4, false>(SC, a, s, st);
+    if (s.LS == 2) launch_ei_form<KC, 2>(SC, a, s, form, st);
+    else launch_ei_form<KC, 4>(SC, a, s, form, st);
 }
 
 FastShape ei_shape(int N, int LS, bool bc) {
@@ -937,18 +1008,18 @@ FastShape ei_shape(int N, int LS, bool bc) {
     return s;
 }
 
-hipError_t launch_sim_ei(const float* SC, const SimArgs& a, int LS, bool bc, hipStream_t st) {
-    const FastShape s = ei_shape(a.N, LS, bc);
+hipError_t launch_sim_ei(const float* SC, const SimArgs& a, int LS, int form, hipStream_t st) {
+    const FastShape s = ei_shape(a.N, LS, form == 1);
     if (!s.ok) return hipErrorInvalidValue;
     switch (s.KC) {
-        case 8: launch_ei_kc<8>(SC, a, s, bc, st); break;
-        case 16: launch_ei_kc<16>(SC, a, s, bc, st); break;
-        case 28: launch_ei_kc<28>(SC, a, s, bc, st); break;
-        case 32: launch_ei_kc<32>(SC, a, s, bc, st); break;
-        case 52: launch_ei_kc<52>(SC, a, s, bc, st); break;
-        case 64: launch_ei_kc<64>(SC, a, s, bc, st); break;
-        case 100: launch_ei_kc<100>(SC, a, s, bc, st); break;
-        default: launch_ei_kc<128>(SC, a, s, bc, st); break;
+        case 8: launch_ei_kc<8>(SC, a, s, form, st); break;
+        case 16: launch_ei_kc<16>(SC, a, s, form, st); break;
+        case 28: launch_ei_kc<28>(SC, a, s, form, st); break;
+        case 32: launch_ei_kc<32>(SC, a, s, form, st); break;
+        case 52: launch_ei_kc<52>(SC, a, s, form, st); break;
+        case 64: launch_ei_kc<64>(SC, a, s, form, st); break;
+        case 100: launch_ei_kc<100>(SC, a, s, form, st); break;
+        default: launch_ei_kc<128>(SC, a, s, form, st); break;
     }
     return hipGetLastError();
 }

@@ -166,13 +166,16 @@ def test_f32_summary_stats_within_bounds(oracle_mod, case, ref, kernel):
     assert d_ks <= 0.01
 
 
-@pytest.mark.parametrize("ls", [2, 4])
-def test_f32_ei_split_kernel_within_bounds(oracle_mod, case, ref, ls, monkeypatch):
+@pytest.mark.parametrize("ls,form", [(2, 0), (4, 0), (2, 2), (4, 2), (2, 3), (4, 3)])
+def test_f32_ei_split_kernel_within_bounds(oracle_mod, case, ref, ls, form, monkeypatch):
     """The FP32 kernel that advances the E and I pools on separate lanes
     (partner state exchanged by lane swaps) against the FP64 oracle: the same
     FIC trials, summary statistics within bounds, and early BOLD close to the
-    FP64 trajectory (a wrong partner lane or pool constant would not be)."""
+    FP64 trajectory (a wrong partner lane or pool constant would not be).
+    form 0: the update as written; 2 / 3: the short-chain update (products
+    of the pool constants folded, four / one partial sums in the dot)."""
     monkeypatch.setenv("BNM_EI_LS", str(ls))
+    monkeypatch.setenv("BNM_EI_FORM", str(form))
     cfg = _cfg(case, bnm.PREC_F32, bnm.KERNEL_FAST)
     with bnm.Context(cfg) as c:
         c.set_sc(case["SC"])
@@ -184,6 +187,8 @@ def test_f32_ei_split_kernel_within_bounds(oracle_mod, case, ref, ls, monkeypatc
         g = c.outputs()
     print("kernel", name)
     assert name.startswith("split_ei_f32(LS=%d" % ls)
+    tag = {0: "", 2: ",short4", 3: ",short1"}[form]
+    assert name.endswith(tag + ")") and (form or ",short" not in name)
     O = oracle_mod
     d_re = np.max(np.abs(g["ext"][:, 2] - ref["ext"][:, 2]))
     d_ext = np.max(np.abs(g["ext"] - ref["ext"]), axis=(0, 2))
