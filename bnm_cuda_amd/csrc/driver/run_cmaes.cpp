@@ -32,6 +32,10 @@
 //    full; when the rank correlation of predicted and simulated cost of the
 //    previous generation falls below BNM_SURR_MIN_RHO, the next generation is
 //    the plain CMA-ES sample (fallback).
+//  * BNM_SURR_MODEL=1 ranks the pool with a local linear-quadratic model
+//    instead (ridge least squares over the last BNM_SURR_LQ_POINTS simulated
+//    points, default max(2 x terms, 2 x lambda)); 2 ranks by the sum of the
+//    MLP's and the quadratic model's ranks.
 //  * every reported cost is a full simulation; the best point is re-run with
 //    the plain method (analytical FIC init) at the end, as in the reference.
 #include <sys/stat.h>
@@ -161,6 +165,94 @@ struct Scaler {
         for (size_t i = 0; i < X.size(); ++i) X[i] = (float)((X[i] - mu[i % d]) / sd[i % d]);
     }
 };
+
+// Local linear-quadratic cost model for pre-screening (the surrogate of
+// Hansen's lq-CMA-ES, 2019): cost ~ full quadratic in the unit-box
+// coordinates, centred and scaled on the fitted points, by ridge least
+// squares; linear when there are too few points for the quadratic terms.
+struct LqModel {
+    int D = 0, nf = 0;
+    bool ok = false;
+    std::vector<double> mu, sd, beta;
+    void features(const double* x, std::vector<double>& f) const {
+        f.assign(nf, 0.0);
+        f[0] = 1.0;
+        std::vector<double> z(D);
+        for (int k = 0; k < D; ++k) z[k] = (x[k] - mu[k]) / sd[k];
+        for (int k = 0; k < D; ++k) f[1 + k] = z[k];
+        if (nf > 1 + D) {
+            int q = 1 + D;
+            for (int a = 0; a < D; ++a)
+                for (int b = a; b < D; ++b) f[q++] = z[a] * z[b];
+        }
+    }
+    // rows x[i*D..], targets y[i]
+    void fit(const std::vector<double>& x, const std::vector<double>& y, int d) {
+        D = d;
+        const int n = (int)y.size();
+        const int nq = 1 + D + D * (D + 1) / 2;
+        nf = n >= nq + D ? nq : 1 + D;
+        ok = n > nf;
+        if (!ok) return;
+        mu.assign(D, 0.0);
+        sd.assign(D, 0.0);
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < D; ++k) mu[k] += x[(size_t)i * D + k] / n;
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < D; ++k) sd[k] += std::pow(x[(size_t)i * D + k] - mu[k], 2) / n;
+        for (int k = 0; k < D; ++k) sd[k] = std::sqrt(sd[k]) + 1e-12;
+        std::vector<double> A((size_t)nf * nf, 0.0), r(nf, 0.0), f;
+        for (int i = 0; i < n; ++i) {
+            features(&x[(size_t)i * D], f);
+            for (int a = 0; a < nf; ++a) {
+                r[a] += f[a] * y[i];
+                for (int b = 0; b < nf; ++b) A[(size_t)a * nf + b] += f[a] * f[b];
+            }
+        }
+        double tr = 0;
+        for (int a = 0; a < nf; ++a) tr += A[(size_t)a * nf + a];
+        for (int a = 0; a < nf; ++a) A[(size_t)a * nf + a] += 1e-8 * tr / nf;
+        // Cholesky solve
+        for (int j = 0; j < nf; ++j) {
+            double dsum = A[(size_t)j * nf + j];
+            for (int k = 0; k < j; ++k) dsum -= A[(size_t)j * nf + k] * A[(size_t)j * nf + k];
+            if (!(dsum > 0)) { ok = false; return; }
+            const double l = std::sqrt(dsum);
+            A[(size_t)j * nf + j] = l;
+            for (int i = j + 1; i < nf; ++i) {
+                double v = A[(size_t)i * nf + j];
+                for (int k = 0; k < j; ++k) v -= A[(size_t)i * nf + k] * A[(size_t)j * nf + k];
+                A[(size_t)i * nf + j] = v / l;
+            }
+        }
+        beta = r;
+        for (int i = 0; i < nf; ++i) {
+            for (int k = 0; k < i; ++k) beta[i] -= A[(size_t)i * nf + k] * beta[k];
+            beta[i] /= A[(size_t)i * nf + i];
+        }
+        for (int i = nf - 1; i >= 0; --i) {
+            for (int k = i + 1; k < nf; ++k) beta[i] -= A[(size_t)k * nf + i] * beta[k];
+            beta[i] /= A[(size_t)i * nf + i];
+        }
+    }
+    double predict(const double* x) const {
+        std::vector<double> f;
+        features(x, f);
+        double v = 0;
+        for (int a = 0; a < nf; ++a) v += beta[a] * f[a];
+        return v;
+    }
+};
+
+// ranks (0 = smallest) of v
+std::vector<double> ranks(const std::vector<double>& v) {
+    std::vector<int> idx(v.size());
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return v[a] < v[b]; });
+    std::vector<double> r(v.size());
+    for (size_t i = 0; i < idx.size(); ++i) r[idx[i]] = (double)i;
+    return r;
+}
 
 }  // namespace
 
@@ -468,6 +560,11 @@ int main(int argc, char* argv[]) {
     std::vector<int> fit_gen;            // generation of each row
     Scaler fit_xs, fit_ys;
     const int window = std::max(1, env_i("BNM_SURR_WINDOW", 6));
+    // pre-screening model: 0 the MLP, 1 the local linear-quadratic model,
+    // 2 both (candidates ranked by the sum of their two ranks)
+    const int surr_model = env_i("BNM_SURR_MODEL", 0);
+    const int lq_points = env_i("BNM_SURR_LQ_POINTS", 0);   // 0: max(2 * terms, 2 * lambda)
+    LqModel lq;
     if (surrogate) {
         fit_net = bnm_mlp_create(D, mlp_hidden, mlp_depth, KF, 1, 13, &st);
         if (!fit_net) die(std::string("MLP: ") + bnm_mlp_last_error());
@@ -524,16 +621,27 @@ int main(int argc, char* argv[]) {
         both(log, "Iteration %d\n", it);
         std::vector<double> pred_cost;
         used_surr = false;
-        if (surrogate && it > warmup && last_rho >= min_rho && pool_factor > 1) {
+        if (surrogate && it > warmup && last_rho >= min_rho && pool_factor > 1 && (surr_model == 0 || lq.ok)) {
             // pre-screen a pool of pool_factor x lambda candidates
             const int n_pool = pool_factor * lambda;
             std::vector<double> pool((size_t)n_pool * D);
             check(bnm_cmaes_ask_pool(es, n_pool, 7919u * it + (unsigned)SeedMW, pool.data()), "ask_pool");
-            std::vector<float> px(pool.begin(), pool.end()), py((size_t)n_pool * KF);
-            fit_xs.apply(px, D);
-            check(bnm_mlp_predict(fit_net, px.data(), n_pool, py.data()), "surrogate predict");
-            std::vector<double> c(n_pool);
-            for (int i = 0; i < n_pool; ++i) c[i] = fit_ys.mu[KF - 1] + fit_ys.sd[KF - 1] * py[(size_t)i * KF + KF - 1];
+            std::vector<double> c(n_pool), cl(n_pool);
+            if (surr_model != 1) {
+                std::vector<float> px(pool.begin(), pool.end()), py((size_t)n_pool * KF);
+                fit_xs.apply(px, D);
+                check(bnm_mlp_predict(fit_net, px.data(), n_pool, py.data()), "surrogate predict");
+                for (int i = 0; i < n_pool; ++i)
+                    c[i] = fit_ys.mu[KF - 1] + fit_ys.sd[KF - 1] * py[(size_t)i * KF + KF - 1];
+            }
+            if (surr_model != 0) {
+                for (int i = 0; i < n_pool; ++i) cl[i] = lq.predict(&pool[(size_t)i * D]);
+                if (surr_model == 1) c = cl;
+                else {
+                    const std::vector<double> ra = ranks(c), rb = ranks(cl);
+                    for (int i = 0; i < n_pool; ++i) c[i] = ra[i] + rb[i];
+                }
+            }
             std::vector<int> idx(n_pool);
             std::iota(idx.begin(), idx.end(), 0);
             std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return c[a] < c[b]; });
@@ -614,11 +722,26 @@ int main(int argc, char* argv[]) {
             fit_ys.fit(wy, n, KF);
             fit_xs.apply(wx, D);
             fit_ys.apply(wy, KF);
-            check(bnm_mlp_set_data(fit_net, wx.data(), wy.data(), n), "surrogate data");
-            const int steps = env_i("BNM_SURR_FIT_STEPS", 1500);
-            const int batch = std::min(n, 128);
-            const int epochs = std::max(1, steps / std::max(1, n / batch));
-            check(bnm_mlp_train(fit_net, epochs, batch, 1e-3f, 17u + it, nullptr), "surrogate train");
+            if (surr_model != 1) {
+                check(bnm_mlp_set_data(fit_net, wx.data(), wy.data(), n), "surrogate data");
+                const int steps = env_i("BNM_SURR_FIT_STEPS", 1500);
+                const int batch = std::min(n, 128);
+                const int epochs = std::max(1, steps / std::max(1, n / batch));
+                check(bnm_mlp_train(fit_net, epochs, batch, 1e-3f, 17u + it, nullptr), "surrogate train");
+            }
+            if (surr_model != 0) {
+                // the most recent points whose simulations produced a cost
+                const int nq = 1 + D + D * (D + 1) / 2;
+                const int want = lq_points > 0 ? lq_points : std::max(2 * nq, 2 * lambda);
+                std::vector<double> lx, ly;
+                for (size_t i = fit_gen.size(); i-- > 0 && (int)ly.size() < want;) {
+                    const double yv = fit_Y[i * KF + KF - 1];
+                    if (!(yv < cfg.max_cost)) continue;
+                    ly.push_back(yv);
+                    lx.insert(lx.end(), &fit_X[i * D], &fit_X[i * D] + D);
+                }
+                lq.fit(lx, ly, D);
+            }
             if (!used_surr) last_rho = 1.0;   // the first fit enables pre-screening
         }
         const int stopped = bnm_cmaes_tell(es, raw.data(), penalized.data());
