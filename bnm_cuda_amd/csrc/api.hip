@@ -63,7 +63,8 @@ struct Variant {
                 // 4 split with the E/I pools on separate lanes (FP32; mode = form:
                 //   0 as written, 1 S_E by DPP row broadcast, 2 / 3 short-chain
                 //   update with four / one partial sums),
-                // 5 stream with register/LDS-resident SC (FP32)
+                // 5 stream with register/LDS-resident SC (FP32; mode 1: S_E by
+                //   DPP row broadcast)
     int ls, ks, mode;
 };
 
@@ -116,7 +117,7 @@ struct bnm_ctx {
     Variant variant{0, 4, 1, 0};
     hipGraphExec_t graph_exec = nullptr;
     bool have_sc = false, have_noise = false, have_emp = false, have_params = false;
-    int fast_ls = 0, fast_ks = 0, stream_ks = 0, force_stream = 0, no_ei = 0, ei_ls = 0, ei_bc = 0, ei_form = -1, no_hyb = 0, hyb_ks = 0;
+    int fast_ls = 0, fast_ks = 0, stream_ks = 0, force_stream = 0, no_ei = 0, ei_ls = 0, ei_bc = 0, ei_form = -1, no_hyb = 0, hyb_ks = 0, hyb_bc = -1, hyb_rr = -1;
 };
 
 extern "C" {
@@ -462,6 +463,10 @@ bnm_ctx* bnm_ctx_create(const bnm_config* cfg, int* status) {
     x->ei_bc = env_int("BNM_EI_BC", 0);
     // BNM_EI_FORM pins the E/I update form (0 as written, 2 / 3 short chain)
     x->ei_form = env_int("BNM_EI_FORM", -1);
+    // BNM_HYB_BC: 0 / 1 restricts the hybrid kernel to LDS / DPP-broadcast S_E
+    x->hyb_bc = env_int("BNM_HYB_BC", -1);
+    // BNM_HYB_RR forces the hybrid kernel's register part (values per lane)
+    x->hyb_rr = env_int("BNM_HYB_RR", -1);
     x->no_hyb = env_int("BNM_NO_HYB", 0);
     x->hyb_ks = env_int("BNM_HYB_KS", 0);
     {
@@ -595,7 +600,7 @@ static bool variant_ok(const bnm_ctx* x, const Variant& v) {
         case 1: return stream_shape(x->N, v.ks, el).ok;
         case 2: return !x->f64 && mfma_shape(x->N, v.mode).ok;
         case 4: return !x->f64 && ei_shape(x->N, v.ls, v.mode == 1).ok;
-        case 5: return !x->f64 && hyb_shape(x->N, v.ks, x->lds_max).ok;
+        case 5: return !x->f64 && hyb_shape(x->N, v.ks, x->lds_max, x->hyb_rr).ok;
         default: return true;
     }
 }
@@ -618,9 +623,9 @@ static std::string variant_name(const bnm_ctx* x, const Variant& v) {
         std::snprintf(name, sizeof name, "mfma_%s(sims/block=16,waves=%d,KQ=%d,threads=%d)",
                       mn[v.mode], s.waves, s.KQ, s.threads);
     } else if (v.kind == 5) {
-        const FastShape g = hyb_shape(x->N, v.ks, x->lds_max);
-        std::snprintf(name, sizeof name, "hyb_f32(KS=%d,NW=%d,kc=%d,regs=%d,lds=%d,streamed=%d,threads=%d)", g.KS,
-                      g.NW, g.kc, g.KC, g.SS, g.kc - g.KC - g.SS, g.threads);
+        const FastShape g = hyb_shape(x->N, v.ks, x->lds_max, x->hyb_rr);
+        std::snprintf(name, sizeof name, "hyb_f32(KS=%d,NW=%d,kc=%d,regs=%d,lds=%d,streamed=%d,threads=%d%s)", g.KS,
+                      g.NW, g.kc, g.KC, g.SS, g.kc - g.KC - g.SS, g.threads, v.mode == 1 ? ",dpp" : "");
     } else if (v.kind == 4) {
         const FastShape s = ei_shape(x->N, v.ls, v.mode == 1);
         static const char* fm[] = {"", ",dpp", ",short4", ",short1"};
@@ -642,7 +647,7 @@ static hipError_t launch_variant(bnm_ctx* x, const Variant& v, const SimArgs& a,
                           : launch_sim_stream<float>(x->SCt32, a, v.ks, st);
         case 2: return launch_sim_mfma(x->SC32, a, v.mode, st);
         case 4: return launch_sim_ei(x->SC32, a, v.ls, v.mode, st);
-        case 5: return launch_sim_hyb(x->SCc32, a, v.ks, x->lds_max, st);
+        case 5: return launch_sim_hyb(x->SCc32, a, v.ks, x->lds_max, x->hyb_rr, v.mode == 1, st);
         default:
             return x->f64 ? launch_sim_plain<double>(x->SC64, a, st)
                           : launch_sim_plain<float>(x->SC32, a, st);
@@ -668,7 +673,7 @@ static std::vector<Variant> candidates(const bnm_ctx* x) {
         return out;
     }
     if (x->hyb_ks > 0 && !x->f64) {
-        add({5, 1, x->hyb_ks, 0});
+        add({5, 1, x->hyb_ks, x->hyb_bc == 1 ? 1 : 0});
         if (!out.empty()) return out;
     }
     if (x->ei_ls > 0 && !x->f64) {
@@ -697,11 +702,15 @@ static std::vector<Variant> candidates(const bnm_ctx* x) {
     if (out.empty() || x->force_stream) {
         if (x->stream_ks > 0) {
             add({1, 1, x->stream_ks, 0});
-            if (!x->f64 && !x->no_hyb) add({5, 1, x->stream_ks, 0});
+            if (!x->f64 && !x->no_hyb)
+                for (int m : {0, 1})
+                    if (x->hyb_bc < 0 || x->hyb_bc == m) add({5, 1, x->stream_ks, m});
         } else {
             for (int k : {16, 8, 4, 2, 1}) add({1, 1, k, 0});
             if (!x->f64 && !x->no_hyb)
-                for (int k : {8, 4, 2, 1}) add({5, 1, k, 0});
+                for (int k : {8, 4, 2, 1})
+                    for (int m : {0, 1})
+                        if (x->hyb_bc < 0 || x->hyb_bc == m) add({5, 1, k, m});
         }
     }
     // the MFMA kernel implements FP32 too; the tuner may pick it for large populations

@@ -32,9 +32,12 @@ FastShape stream_shape(int N, int KS, size_t elem);
 constexpr size_t kSCtPad = 256;
 // FP32 streaming kernel with register- and LDS-resident SC (N > 128); the
 // shape's KC field is the register part, SS the LDS part (values per lane)
-FastShape hyb_shape(int N, int KS, size_t lds_bytes);
+// (rr >= 0: register part forced, a multiple of 16 up to 160)
+FastShape hyb_shape(int N, int KS, size_t lds_bytes, int rr = -1);
 // SCc: chunked layout SCc[((k/16)*N + j)*16 + k%16] = SC[j][k], zero for k >= N
-hipError_t launch_sim_hyb(const float* SCc, const SimArgs& a, int KS, size_t lds_bytes, hipStream_t st);
+// bc: S_E by DPP row broadcast instead of 16-byte LDS broadcast reads
+hipError_t launch_sim_hyb(const float* SCc, const SimArgs& a, int KS, size_t lds_bytes, int rr, bool bc,
+                          hipStream_t st);
 // FP32 split kernel with the node update divided between lanes by population
 FastShape ei_shape(int N, int LS, bool bc = false);
 // form: 0 as written, 1 DPP row broadcast (LS = 4, N <= 128), 2 / 3 short-chain

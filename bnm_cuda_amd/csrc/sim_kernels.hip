@@ -739,7 +739,13 @@ __global__ __launch_bounds__(1024) void k_sim_split_g(const R* __restrict__ SCt,
 // the plain streaming kernel at N = 400 (640 KB of SC per substep);
 // residency cuts the streamed part to what LDS and registers cannot hold.
 // ---------------------------------------------------------------------------
-template <int RR, int MAXT>
+//
+// BC: S_E reaches the dot by DPP row broadcast.  Every lane of a wave needs
+// the same S_E[k]; for each 16-k chunk lane q of every 16-lane row reads
+// S_E[k + q] (one 4-byte LDS read, 256 bytes per wave instead of 4 KB of
+// 16-byte broadcast reads) and v_fmac_f32_dpp row_newbcast:q hands it to the
+// whole row.  Same accumulation order as the LDS form.
+template <int RR, int MAXT, bool BC>
 __global__ __launch_bounds__(MAXT) void k_sim_hyb(const float* __restrict__ SCc, SimArgs a, int NW, int KS,
                                                   int kc, int LL) {
     constexpr int CH = 16;
@@ -806,45 +812,63 @@ __global__ __launch_bounds__(MAXT) void k_sim_hyb(const float* __restrict__ SCc,
             float w0[CH], w1[CH];
             if (nch > 0) chunk(gb, w0);
             const float4* Sr = reinterpret_cast<const float4*>(Sb + buf * ldS + k0);
+            const float* Sq = Sb + buf * ldS + k0 + (lane & 15);   // BC: this lane's S_E of each chunk
             float acc[4] = {0, 0, 0, 0};
-#pragma unroll
-            for (int q4 = 0; q4 < RR / 4; ++q4) {
-                const float4 v = Sr[q4];
-                acc[0] = fmaf(wr[4 * q4 + 0], v.x, acc[0]);
-                acc[1] = fmaf(wr[4 * q4 + 1], v.y, acc[1]);
-                acc[2] = fmaf(wr[4 * q4 + 2], v.z, acc[2]);
-                acc[3] = fmaf(wr[4 * q4 + 3], v.w, acc[3]);
-            }
-            for (int q4 = 0; q4 < LL / 4; ++q4) {
-                const float4 v = Sr[RR / 4 + q4];
-                const float4 w = SCl[(size_t)q4 * T + tid];
-                acc[0] = fmaf(w.x, v.x, acc[0]);
-                acc[1] = fmaf(w.y, v.y, acc[1]);
-                acc[2] = fmaf(w.z, v.z, acc[2]);
-                acc[3] = fmaf(w.w, v.w, acc[3]);
-            }
-            const float4* Sg = Sr + (RR + LL) / 4;
-            for (int c = 0; c < nch; c += 2) {
-                if (c + 1 < nch) chunk(gb + c + 1, w1);
-#pragma unroll
-                for (int q4 = 0; q4 < CH / 4; ++q4) {
-                    const float4 v = Sg[c * (CH / 4) + q4];
-                    acc[0] = fmaf(w0[4 * q4 + 0], v.x, acc[0]);
-                    acc[1] = fmaf(w0[4 * q4 + 1], v.y, acc[1]);
-                    acc[2] = fmaf(w0[4 * q4 + 2], v.z, acc[2]);
-                    acc[3] = fmaf(w0[4 * q4 + 3], v.w, acc[3]);
-                }
-                if (c + 2 < nch) chunk(gb + c + 2, w0);
-                if (c + 1 < nch) {
+            // the 16 products of one chunk: w[0..15] times S_E[kq .. kq+15]
+            auto dot16 = [&](const float* w, int kq) {
+                if constexpr (BC) {
+                    const float src = Sq[kq];
+                    bcast_fma4<0>(acc, src, w[0], w[1], w[2], w[3]);
+                    bcast_fma4<1>(acc, src, w[4], w[5], w[6], w[7]);
+                    bcast_fma4<2>(acc, src, w[8], w[9], w[10], w[11]);
+                    bcast_fma4<3>(acc, src, w[12], w[13], w[14], w[15]);
+                } else {
 #pragma unroll
                     for (int q4 = 0; q4 < CH / 4; ++q4) {
-                        const float4 v = Sg[(c + 1) * (CH / 4) + q4];
-                        acc[0] = fmaf(w1[4 * q4 + 0], v.x, acc[0]);
-                        acc[1] = fmaf(w1[4 * q4 + 1], v.y, acc[1]);
-                        acc[2] = fmaf(w1[4 * q4 + 2], v.z, acc[2]);
-                        acc[3] = fmaf(w1[4 * q4 + 3], v.w, acc[3]);
+                        const float4 v = Sr[kq / 4 + q4];
+                        acc[0] = fmaf(w[4 * q4 + 0], v.x, acc[0]);
+                        acc[1] = fmaf(w[4 * q4 + 1], v.y, acc[1]);
+                        acc[2] = fmaf(w[4 * q4 + 2], v.z, acc[2]);
+                        acc[3] = fmaf(w[4 * q4 + 3], v.w, acc[3]);
                     }
                 }
+            };
+            if constexpr (BC) {
+#pragma unroll
+                for (int c = 0; c < RR / CH; ++c) dot16(wr + c * CH, c * CH);
+                for (int c = 0; c < LL / CH; ++c) {
+                    float w[CH];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const float4 v = SCl[(size_t)(c * 4 + q) * T + tid];
+                        w[4 * q + 0] = v.x; w[4 * q + 1] = v.y; w[4 * q + 2] = v.z; w[4 * q + 3] = v.w;
+                    }
+                    dot16(w, RR + c * CH);
+                }
+            } else {
+#pragma unroll
+                for (int q4 = 0; q4 < RR / 4; ++q4) {
+                    const float4 v = Sr[q4];
+                    acc[0] = fmaf(wr[4 * q4 + 0], v.x, acc[0]);
+                    acc[1] = fmaf(wr[4 * q4 + 1], v.y, acc[1]);
+                    acc[2] = fmaf(wr[4 * q4 + 2], v.z, acc[2]);
+                    acc[3] = fmaf(wr[4 * q4 + 3], v.w, acc[3]);
+                }
+                for (int q4 = 0; q4 < LL / 4; ++q4) {
+                    const float4 v = Sr[RR / 4 + q4];
+                    const float4 w = SCl[(size_t)q4 * T + tid];
+                    acc[0] = fmaf(w.x, v.x, acc[0]);
+                    acc[1] = fmaf(w.y, v.y, acc[1]);
+                    acc[2] = fmaf(w.z, v.z, acc[2]);
+                    acc[3] = fmaf(w.w, v.w, acc[3]);
+                }
+            }
+            const int kg = RR + LL;
+            for (int c = 0; c < nch; c += 2) {
+                if (c + 1 < nch) chunk(gb + c + 1, w1);
+                dot16(w0, kg + c * CH);
+                if (c + 2 < nch) chunk(gb + c + 2, w0);
+                if (c + 1 < nch) dot16(w1, kg + (c + 1) * CH);
             }
             float c = (acc[0] + acc[1]) + (acc[2] + acc[3]);
             if (KS > 1) {
@@ -1047,14 +1071,18 @@ hipError_t launch_sim_stream(const R* SCt, const SimArgs& a, int KS, hipStream_t
     return hipGetLastError();
 }
 
-// Hybrid shape (at most 512 threads): RR register-resident values per lane
-// (96 with two waves per SIMD, 160 with one), then as many LDS
-// values as `lds_bytes` holds, rounded so the streamed rest is whole chunks.
-FastShape hyb_shape(int N, int KS, size_t lds_bytes) {
+// largest workgroup a register part of RR values per lane compiles for
+// without spilling (512 VGPRs per SIMD lane: 256 per wave at two waves)
+constexpr int hyb_maxt(int RR) { return RR == 0 ? 1024 : RR <= 128 ? 512 : 256; }
+
+// Hybrid shape: RR register-resident values per lane (128 with two waves
+// per SIMD, 160 with one; rr >= 0 overrides), then as many LDS values as
+// `lds_bytes` holds, rounded so the streamed rest is whole chunks.
+FastShape hyb_shape(int N, int KS, size_t lds_bytes, int rr) {
     FastShape s = stream_shape(N, KS, sizeof(float));
     // RR: the register part the workgroup size leaves room for (one or two
     // waves per SIMD), capped by the lane's k-range
-    s.KC = s.threads <= 256 ? 160 : s.threads <= 512 ? 96 : 0;
+    s.KC = rr >= 0 ? rr : s.threads <= 256 ? 160 : s.threads <= 512 ? 128 : 0;
     while (s.KC > s.kc) s.KC -= 16;
     const size_t base = (2 * (size_t)KS * s.kc + (size_t)KS * s.NW * 64) * sizeof(float);
     int LL = 0;
@@ -1063,27 +1091,37 @@ FastShape hyb_shape(int N, int KS, size_t lds_bytes) {
     s.SS = LL;
     s.smem = base + (size_t)LL * s.threads * sizeof(float);
     s.ok = s.threads <= 1024 && s.smem <= lds_bytes && (s.kc - s.KC - LL) % 16 == 0 && s.KC % 16 == 0 &&
-           (s.KC == 160 || s.KC == 96 || s.KC == 64 || s.KC == 32 || s.KC == 16 || s.KC == 0);
+           s.KC >= 0 && s.KC <= 160 && s.threads <= hyb_maxt(s.KC);
     return s;
 }
 
 template <int RR>
-static void launch_hyb(const float* SCc, const SimArgs& a, const FastShape& s, hipStream_t st) {
-    constexpr int MAXT = RR == 0 ? 1024 : RR <= 96 ? 512 : 256;
-    hipLaunchKernelGGL((k_sim_hyb<RR, MAXT>), dim3(a.B), dim3(s.threads), s.smem, st, SCc, a, s.NW, s.KS,
-                       s.kc, s.SS);
+static void launch_hyb(const float* SCc, const SimArgs& a, const FastShape& s, bool bc, hipStream_t st) {
+    constexpr int MAXT = hyb_maxt(RR);
+    if (bc)
+        hipLaunchKernelGGL((k_sim_hyb<RR, MAXT, true>), dim3(a.B), dim3(s.threads), s.smem, st, SCc, a, s.NW, s.KS,
+                           s.kc, s.SS);
+    else
+        hipLaunchKernelGGL((k_sim_hyb<RR, MAXT, false>), dim3(a.B), dim3(s.threads), s.smem, st, SCc, a, s.NW,
+                           s.KS, s.kc, s.SS);
 }
 
-hipError_t launch_sim_hyb(const float* SCc, const SimArgs& a, int KS, size_t lds_bytes, hipStream_t st) {
-    const FastShape s = hyb_shape(a.N, KS, lds_bytes);
+hipError_t launch_sim_hyb(const float* SCc, const SimArgs& a, int KS, size_t lds_bytes, int rr, bool bc,
+                          hipStream_t st) {
+    const FastShape s = hyb_shape(a.N, KS, lds_bytes, rr);
     if (!s.ok) return hipErrorInvalidValue;
     switch (s.KC) {
-        case 0: launch_hyb<0>(SCc, a, s, st); break;
-        case 16: launch_hyb<16>(SCc, a, s, st); break;
-        case 32: launch_hyb<32>(SCc, a, s, st); break;
-        case 64: launch_hyb<64>(SCc, a, s, st); break;
-        case 96: launch_hyb<96>(SCc, a, s, st); break;
-        case 160: launch_hyb<160>(SCc, a, s, st); break;
+        case 0: launch_hyb<0>(SCc, a, s, bc, st); break;
+        case 16: launch_hyb<16>(SCc, a, s, bc, st); break;
+        case 32: launch_hyb<32>(SCc, a, s, bc, st); break;
+        case 48: launch_hyb<48>(SCc, a, s, bc, st); break;
+        case 64: launch_hyb<64>(SCc, a, s, bc, st); break;
+        case 80: launch_hyb<80>(SCc, a, s, bc, st); break;
+        case 96: launch_hyb<96>(SCc, a, s, bc, st); break;
+        case 112: launch_hyb<112>(SCc, a, s, bc, st); break;
+        case 128: launch_hyb<128>(SCc, a, s, bc, st); break;
+        case 144: launch_hyb<144>(SCc, a, s, bc, st); break;
+        case 160: launch_hyb<160>(SCc, a, s, bc, st); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

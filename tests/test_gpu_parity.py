@@ -427,6 +427,38 @@ def test_hybrid_resident_sc_kernel_vs_f64(monkeypatch, N_big, ks):
     assert rel <= 1e-4 and d_re <= 0.05 and d_fc <= 1e-3
 
 
+@pytest.mark.parametrize("N_big,ks", [(200, 1), (200, 2), (400, 1), (400, 2)])
+def test_hybrid_dpp_broadcast_bit_identical(monkeypatch, N_big, ks):
+    """The hybrid kernel with S_E spread by DPP row broadcast against the
+    same kernel reading S_E by 16-byte LDS broadcast: the same products in
+    the same order, so every output is bit-identical."""
+    print("case: hybrid dpp broadcast N", N_big, "KS", ks, flush=True)
+    B4, T4 = 4, 20_000
+    SC = W.synthetic_sc(N_big, 0.3, 23)
+    g, ee, ei = W.population(B4, 24)
+    noise = bnm.make_noise_chunked(25, (T4 + 1) * 20 * N_big)
+    rng = np.random.default_rng(26)
+    emp = (rng.uniform(-0.2, 0.8, bnm.n_pairs(N_big, False)), rng.uniform(-0.5, 1, 40))
+    monkeypatch.setenv("BNM_HYB_KS", str(ks))
+    out = {}
+    for bc in (0, 1):
+        monkeypatch.setenv("BNM_HYB_BC", str(bc))
+        cfg = bnm.default_config(nodes=N_big, n_sims=B4, time_steps=T4, bold_tr=1000, n_vols_remove=0,
+                                 window_size=10, window_step=2, exc_interhemispheric=0,
+                                 noise_period_ms=T4 + 1, precision=bnm.PREC_F32, kernel=bnm.KERNEL_FAST,
+                                 fc_precision=bnm.FC_F32)
+        with bnm.Context(cfg) as c:
+            c.set_sc(SC); c.set_noise(noise); c.set_empirical(*emp)
+            c.set_params(g, W.expand_homogeneous(ee, N_big), W.expand_homogeneous(ei, N_big))
+            c.run()
+            name = c.kernel_name()
+            print(name, c.timings())
+            assert name.startswith("hyb_f32(KS=%d" % ks) and name.endswith(",dpp)") == bool(bc)
+            out[bc] = c.outputs()
+    for k in ("bold", "ext", "wIE_fic", "fic_trials", "fc", "fcd", "corr"):
+        np.testing.assert_array_equal(out[0][k], out[1][k], err_msg=k)
+
+
 @pytest.mark.parametrize("exc", [1, 0])
 def test_hemisphere_fcd_matches_window_fc_subsets(oracle_mod, case, ref, exc):
     """save_hemis: left / right FCD = Pearson between windows over the first /
