@@ -64,7 +64,7 @@ struct Variant {
                 //   0 as written, 1 S_E by DPP row broadcast, 2 / 3 short-chain
                 //   update with four / one partial sums),
                 // 5 stream with register/LDS-resident SC (FP32; mode 1: S_E by
-                //   DPP row broadcast)
+                //   DPP row broadcast, 2: and three streamed chunks in flight)
     int ls, ks, mode;
 };
 
@@ -463,7 +463,8 @@ bnm_ctx* bnm_ctx_create(const bnm_config* cfg, int* status) {
     x->ei_bc = env_int("BNM_EI_BC", 0);
     // BNM_EI_FORM pins the E/I update form (0 as written, 2 / 3 short chain)
     x->ei_form = env_int("BNM_EI_FORM", -1);
-    // BNM_HYB_BC: 0 / 1 restricts the hybrid kernel to LDS / DPP-broadcast S_E
+    // BNM_HYB_BC: 0 / 1 / 2 restricts the hybrid kernel to LDS / DPP-broadcast
+    // S_E / DPP broadcast with three streamed chunks in flight
     x->hyb_bc = env_int("BNM_HYB_BC", -1);
     // BNM_HYB_RR forces the hybrid kernel's register part (values per lane)
     x->hyb_rr = env_int("BNM_HYB_RR", -1);
@@ -600,7 +601,7 @@ static bool variant_ok(const bnm_ctx* x, const Variant& v) {
         case 1: return stream_shape(x->N, v.ks, el).ok;
         case 2: return !x->f64 && mfma_shape(x->N, v.mode).ok;
         case 4: return !x->f64 && ei_shape(x->N, v.ls, v.mode == 1).ok;
-        case 5: return !x->f64 && hyb_shape(x->N, v.ks, x->lds_max, x->hyb_rr).ok;
+        case 5: return !x->f64 && hyb_shape(x->N, v.ks, x->lds_max, x->hyb_rr, v.mode).ok;
         default: return true;
     }
 }
@@ -623,9 +624,10 @@ static std::string variant_name(const bnm_ctx* x, const Variant& v) {
         std::snprintf(name, sizeof name, "mfma_%s(sims/block=16,waves=%d,KQ=%d,threads=%d)",
                       mn[v.mode], s.waves, s.KQ, s.threads);
     } else if (v.kind == 5) {
-        const FastShape g = hyb_shape(x->N, v.ks, x->lds_max, x->hyb_rr);
+        const FastShape g = hyb_shape(x->N, v.ks, x->lds_max, x->hyb_rr, v.mode);
         std::snprintf(name, sizeof name, "hyb_f32(KS=%d,NW=%d,kc=%d,regs=%d,lds=%d,streamed=%d,threads=%d%s)", g.KS,
-                      g.NW, g.kc, g.KC, g.SS, g.kc - g.KC - g.SS, g.threads, v.mode == 1 ? ",dpp" : "");
+                      g.NW, g.kc, g.KC, g.SS, g.kc - g.KC - g.SS, g.threads,
+                      v.mode == 2 ? ",dpp,pf3" : v.mode == 1 ? ",dpp" : "");
     } else if (v.kind == 4) {
         const FastShape s = ei_shape(x->N, v.ls, v.mode == 1);
         static const char* fm[] = {"", ",dpp", ",short4", ",short1"};
@@ -647,7 +649,7 @@ static hipError_t launch_variant(bnm_ctx* x, const Variant& v, const SimArgs& a,
                           : launch_sim_stream<float>(x->SCt32, a, v.ks, st);
         case 2: return launch_sim_mfma(x->SC32, a, v.mode, st);
         case 4: return launch_sim_ei(x->SC32, a, v.ls, v.mode, st);
-        case 5: return launch_sim_hyb(x->SCc32, a, v.ks, x->lds_max, x->hyb_rr, v.mode == 1, st);
+        case 5: return launch_sim_hyb(x->SCc32, a, v.ks, x->lds_max, x->hyb_rr, v.mode, st);
         default:
             return x->f64 ? launch_sim_plain<double>(x->SC64, a, st)
                           : launch_sim_plain<float>(x->SC32, a, st);
@@ -673,7 +675,7 @@ static std::vector<Variant> candidates(const bnm_ctx* x) {
         return out;
     }
     if (x->hyb_ks > 0 && !x->f64) {
-        add({5, 1, x->hyb_ks, x->hyb_bc == 1 ? 1 : 0});
+        add({5, 1, x->hyb_ks, x->hyb_bc > 0 ? x->hyb_bc : 0});
         if (!out.empty()) return out;
     }
     if (x->ei_ls > 0 && !x->f64) {
@@ -703,13 +705,13 @@ static std::vector<Variant> candidates(const bnm_ctx* x) {
         if (x->stream_ks > 0) {
             add({1, 1, x->stream_ks, 0});
             if (!x->f64 && !x->no_hyb)
-                for (int m : {0, 1})
+                for (int m : {0, 1, 2})
                     if (x->hyb_bc < 0 || x->hyb_bc == m) add({5, 1, x->stream_ks, m});
         } else {
             for (int k : {16, 8, 4, 2, 1}) add({1, 1, k, 0});
             if (!x->f64 && !x->no_hyb)
                 for (int k : {8, 4, 2, 1})
-                    for (int m : {0, 1})
+                    for (int m : {0, 1, 2})
                         if (x->hyb_bc < 0 || x->hyb_bc == m) add({5, 1, k, m});
         }
     }

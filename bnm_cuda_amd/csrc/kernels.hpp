@@ -33,10 +33,11 @@ constexpr size_t kSCtPad = 256;
 // FP32 streaming kernel with register- and LDS-resident SC (N > 128); the
 // shape's KC field is the register part, SS the LDS part (values per lane)
 // (rr >= 0: register part forced, a multiple of 16 up to 160)
-FastShape hyb_shape(int N, int KS, size_t lds_bytes, int rr = -1);
+FastShape hyb_shape(int N, int KS, size_t lds_bytes, int rr = -1, int form = 0);
 // SCc: chunked layout SCc[((k/16)*N + j)*16 + k%16] = SC[j][k], zero for k >= N
-// bc: S_E by DPP row broadcast instead of 16-byte LDS broadcast reads
-hipError_t launch_sim_hyb(const float* SCc, const SimArgs& a, int KS, size_t lds_bytes, int rr, bool bc,
+// form: 0 S_E by 16-byte LDS broadcast reads, 1 by DPP row broadcast, 2 DPP
+// broadcast with three streamed chunks in flight
+hipError_t launch_sim_hyb(const float* SCc, const SimArgs& a, int KS, size_t lds_bytes, int rr, int form,
                           hipStream_t st);
 // FP32 split kernel with the node update divided between lanes by population
 FastShape ei_shape(int N, int LS, bool bc = false);

@@ -745,9 +745,18 @@ __global__ __launch_bounds__(1024) void k_sim_split_g(const R* __restrict__ SCt,
 // S_E[k + q] (one 4-byte LDS read, 256 bytes per wave instead of 4 KB of
 // 16-byte broadcast reads) and v_fmac_f32_dpp row_newbcast:q hands it to the
 // whole row.  Same accumulation order as the LDS form.
-template <int RR, int MAXT, bool BC>
+//
+// FORM 0: S_E by LDS broadcast; 1: BC; 2: BC with three streamed chunks in
+// flight, the first three issued before the register-resident products and
+// the LDS-resident chunks interleaved between streamed ones, so that each
+// load has about three chunks of work to hide its L2 latency behind (with
+// one chunk ahead, each of the ~12 streamed chunks at N = 400 waits out most
+// of it).  FORM 2 accumulates in another order (LDS chunks between streamed
+// ones).
+template <int RR, int MAXT, int FORM>
 __global__ __launch_bounds__(MAXT) void k_sim_hyb(const float* __restrict__ SCc, SimArgs a, int NW, int KS,
                                                   int kc, int LL) {
+    constexpr bool BC = FORM >= 1;
     constexpr int CH = 16;
     static_assert(RR % CH == 0, "register part must be whole chunks");
     const int sim = blockIdx.x;
@@ -809,8 +818,12 @@ __global__ __launch_bounds__(MAXT) void k_sim_hyb(const float* __restrict__ SCc,
 #pragma unroll 1
         for (int s = 0; s < 10; ++s) {
             const float2 nz = comp ? nzp[(size_t)s * N] : float2{0.0f, 0.0f};
-            float w0[CH], w1[CH];
+            float w0[CH], w1[CH], w2[FORM == 2 ? CH : 1];
             if (nch > 0) chunk(gb, w0);
+            if constexpr (FORM == 2) {
+                if (nch > 1) chunk(gb + 1, w1);
+                if (nch > 2) chunk(gb + 2, w2);
+            }
             const float4* Sr = reinterpret_cast<const float4*>(Sb + buf * ldS + k0);
             const float* Sq = Sb + buf * ldS + k0 + (lane & 15);   // BC: this lane's S_E of each chunk
             float acc[4] = {0, 0, 0, 0};
@@ -833,7 +846,38 @@ __global__ __launch_bounds__(MAXT) void k_sim_hyb(const float* __restrict__ SCc,
                     }
                 }
             };
-            if constexpr (BC) {
+            // LDS-resident chunk c
+            auto lds16 = [&](int c) {
+                float w[CH];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float4 v = SCl[(size_t)(c * 4 + q) * T + tid];
+                    w[4 * q + 0] = v.x; w[4 * q + 1] = v.y; w[4 * q + 2] = v.z; w[4 * q + 3] = v.w;
+                }
+                dot16(w, RR + c * CH);
+            };
+            if constexpr (FORM == 2) {
+#pragma unroll
+                for (int c = 0; c < RR / CH; ++c) dot16(wr + c * CH, c * CH);
+                const int kg = RR + LL, nl = LL / CH;
+                int l = 0;
+                for (int c = 0; c < nch; c += 3) {
+                    dot16(w0, kg + c * CH);
+                    if (c + 3 < nch) chunk(gb + c + 3, w0);
+                    if (l < nl) lds16(l++);
+                    if (c + 1 < nch) {
+                        dot16(w1, kg + (c + 1) * CH);
+                        if (c + 4 < nch) chunk(gb + c + 4, w1);
+                        if (l < nl) lds16(l++);
+                    }
+                    if (c + 2 < nch) {
+                        dot16(w2, kg + (c + 2) * CH);
+                        if (c + 5 < nch) chunk(gb + c + 5, w2);
+                        if (l < nl) lds16(l++);
+                    }
+                }
+                while (l < nl) lds16(l++);
+            } else if constexpr (BC) {
 #pragma unroll
                 for (int c = 0; c < RR / CH; ++c) dot16(wr + c * CH, c * CH);
                 for (int c = 0; c < LL / CH; ++c) {
@@ -863,12 +907,14 @@ __global__ __launch_bounds__(MAXT) void k_sim_hyb(const float* __restrict__ SCc,
                     acc[3] = fmaf(w.w, v.w, acc[3]);
                 }
             }
-            const int kg = RR + LL;
-            for (int c = 0; c < nch; c += 2) {
-                if (c + 1 < nch) chunk(gb + c + 1, w1);
-                dot16(w0, kg + c * CH);
-                if (c + 2 < nch) chunk(gb + c + 2, w0);
-                if (c + 1 < nch) dot16(w1, kg + (c + 1) * CH);
+            if constexpr (FORM != 2) {
+                const int kg = RR + LL;
+                for (int c = 0; c < nch; c += 2) {
+                    if (c + 1 < nch) chunk(gb + c + 1, w1);
+                    dot16(w0, kg + c * CH);
+                    if (c + 2 < nch) chunk(gb + c + 2, w0);
+                    if (c + 1 < nch) dot16(w1, kg + (c + 1) * CH);
+                }
             }
             float c = (acc[0] + acc[1]) + (acc[2] + acc[3]);
             if (KS > 1) {
@@ -1075,14 +1121,15 @@ hipError_t launch_sim_stream(const R* SCt, const SimArgs& a, int KS, hipStream_t
 // without spilling (512 VGPRs per SIMD lane: 256 per wave at two waves)
 constexpr int hyb_maxt(int RR) { return RR == 0 ? 1024 : RR <= 128 ? 512 : 256; }
 
-// Hybrid shape: RR register-resident values per lane (128 with two waves
-// per SIMD, 160 with one; rr >= 0 overrides), then as many LDS values as
-// `lds_bytes` holds, rounded so the streamed rest is whole chunks.
-FastShape hyb_shape(int N, int KS, size_t lds_bytes, int rr) {
+// Hybrid shape: RR register-resident values per lane (with two waves per
+// SIMD 128, or 112 for form 2's third chunk buffer; 160 / 144 with one; rr >= 0
+// overrides), then as many LDS values as `lds_bytes` holds, rounded so the
+// streamed rest is whole chunks.
+FastShape hyb_shape(int N, int KS, size_t lds_bytes, int rr, int form) {
     FastShape s = stream_shape(N, KS, sizeof(float));
     // RR: the register part the workgroup size leaves room for (one or two
     // waves per SIMD), capped by the lane's k-range
-    s.KC = rr >= 0 ? rr : s.threads <= 256 ? 160 : s.threads <= 512 ? 128 : 0;
+    s.KC = rr >= 0 ? rr : s.threads <= 256 ? (form == 2 ? 144 : 160) : s.threads <= 512 ? (form == 2 ? 112 : 128) : 0;
     while (s.KC > s.kc) s.KC -= 16;
     const size_t base = (2 * (size_t)KS * s.kc + (size_t)KS * s.NW * 64) * sizeof(float);
     int LL = 0;
@@ -1092,36 +1139,42 @@ FastShape hyb_shape(int N, int KS, size_t lds_bytes, int rr) {
     s.smem = base + (size_t)LL * s.threads * sizeof(float);
     s.ok = s.threads <= 1024 && s.smem <= lds_bytes && (s.kc - s.KC - LL) % 16 == 0 && s.KC % 16 == 0 &&
            s.KC >= 0 && s.KC <= 160 && s.threads <= hyb_maxt(s.KC);
+    // form 2's third chunk buffer: without spills up to 112 values at two
+    // waves per SIMD and 144 at one, not in the 1024-thread (RR = 0) build
+    if (form == 2) s.ok = s.ok && s.KC >= 16 && s.KC <= (s.threads <= 256 ? 144 : 112);
     return s;
 }
 
 template <int RR>
-static void launch_hyb(const float* SCc, const SimArgs& a, const FastShape& s, bool bc, hipStream_t st) {
+static void launch_hyb(const float* SCc, const SimArgs& a, const FastShape& s, int form, hipStream_t st) {
     constexpr int MAXT = hyb_maxt(RR);
-    if (bc)
-        hipLaunchKernelGGL((k_sim_hyb<RR, MAXT, true>), dim3(a.B), dim3(s.threads), s.smem, st, SCc, a, s.NW, s.KS,
+    if (form == 2)
+        hipLaunchKernelGGL((k_sim_hyb<RR, MAXT, 2>), dim3(a.B), dim3(s.threads), s.smem, st, SCc, a, s.NW, s.KS,
+                           s.kc, s.SS);
+    else if (form == 1)
+        hipLaunchKernelGGL((k_sim_hyb<RR, MAXT, 1>), dim3(a.B), dim3(s.threads), s.smem, st, SCc, a, s.NW, s.KS,
                            s.kc, s.SS);
     else
-        hipLaunchKernelGGL((k_sim_hyb<RR, MAXT, false>), dim3(a.B), dim3(s.threads), s.smem, st, SCc, a, s.NW,
-                           s.KS, s.kc, s.SS);
+        hipLaunchKernelGGL((k_sim_hyb<RR, MAXT, 0>), dim3(a.B), dim3(s.threads), s.smem, st, SCc, a, s.NW, s.KS,
+                           s.kc, s.SS);
 }
 
-hipError_t launch_sim_hyb(const float* SCc, const SimArgs& a, int KS, size_t lds_bytes, int rr, bool bc,
+hipError_t launch_sim_hyb(const float* SCc, const SimArgs& a, int KS, size_t lds_bytes, int rr, int form,
                           hipStream_t st) {
-    const FastShape s = hyb_shape(a.N, KS, lds_bytes, rr);
+    const FastShape s = hyb_shape(a.N, KS, lds_bytes, rr, form);
     if (!s.ok) return hipErrorInvalidValue;
     switch (s.KC) {
-        case 0: launch_hyb<0>(SCc, a, s, bc, st); break;
-        case 16: launch_hyb<16>(SCc, a, s, bc, st); break;
-        case 32: launch_hyb<32>(SCc, a, s, bc, st); break;
-        case 48: launch_hyb<48>(SCc, a, s, bc, st); break;
-        case 64: launch_hyb<64>(SCc, a, s, bc, st); break;
-        case 80: launch_hyb<80>(SCc, a, s, bc, st); break;
-        case 96: launch_hyb<96>(SCc, a, s, bc, st); break;
-        case 112: launch_hyb<112>(SCc, a, s, bc, st); break;
-        case 128: launch_hyb<128>(SCc, a, s, bc, st); break;
-        case 144: launch_hyb<144>(SCc, a, s, bc, st); break;
-        case 160: launch_hyb<160>(SCc, a, s, bc, st); break;
+        case 0: launch_hyb<0>(SCc, a, s, form, st); break;
+        case 16: launch_hyb<16>(SCc, a, s, form, st); break;
+        case 32: launch_hyb<32>(SCc, a, s, form, st); break;
+        case 48: launch_hyb<48>(SCc, a, s, form, st); break;
+        case 64: launch_hyb<64>(SCc, a, s, form, st); break;
+        case 80: launch_hyb<80>(SCc, a, s, form, st); break;
+        case 96: launch_hyb<96>(SCc, a, s, form, st); break;
+        case 112: launch_hyb<112>(SCc, a, s, form, st); break;
+        case 128: launch_hyb<128>(SCc, a, s, form, st); break;
+        case 144: launch_hyb<144>(SCc, a, s, form, st); break;
+        case 160: launch_hyb<160>(SCc, a, s, form, st); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

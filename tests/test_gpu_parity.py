@@ -386,12 +386,15 @@ def test_n400_streamed_f32_vs_f64():
     assert np.abs(a["fc"] - b["fc"]).max() <= 1e-3
 
 
-@pytest.mark.parametrize("N_big,ks", [(200, 1), (200, 2), (200, 4), (400, 1), (400, 2)])
-def test_hybrid_resident_sc_kernel_vs_f64(monkeypatch, N_big, ks):
+@pytest.mark.parametrize("N_big,ks,form", [(200, 1, 0), (200, 2, 0), (200, 4, 0), (400, 1, 0), (400, 2, 0),
+                                           (400, 1, 2), (200, 1, 2)])
+def test_hybrid_resident_sc_kernel_vs_f64(monkeypatch, N_big, ks, form):
     """FP32 kernel with register- and LDS-resident SC and the rest streamed
     (chunked SC layout, buffer loads) against the FP64 streamed kernel on
     the same inputs: same FIC trials, summary statistics within bounds,
-    early BOLD close (a wrong chunk offset or LDS slot would not be)."""
+    early BOLD close (a wrong chunk offset or LDS slot would not be).
+    form 2: three streamed chunks in flight, LDS chunks interleaved."""
+    monkeypatch.setenv("BNM_HYB_BC", str(form))
     B4, T4 = 4, 30_000
     SC = W.synthetic_sc(N_big, 0.3, 17)
     rng = np.random.default_rng(5)
@@ -416,6 +419,7 @@ def test_hybrid_resident_sc_kernel_vs_f64(monkeypatch, N_big, ks):
             print(c.kernel_name(), c.timings())
             if prec == bnm.PREC_F32:
                 assert c.kernel_name().startswith("hyb_f32(KS=%d" % ks)
+                assert c.kernel_name().endswith(",pf3)") == (form == 2)
             out[prec] = c.outputs()
     a, b = out[bnm.PREC_F32], out[bnm.PREC_F64]
     rel = _bold_rel(a["bold"][:, :11], b["bold"][:, :11])
