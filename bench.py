@@ -273,7 +273,8 @@ def main():
         return
     if args.workload in ("uneven", "fcd400"):
         from bnm_cuda_amd import experiments as X
-        res = X.uneven_batch(durations_scale=args.scale) if args.workload == "uneven" else X.fcd_stress()
+        res = (X.uneven_batch(durations_scale=args.scale, fp64=not args.no_fp64, bf16x3=not args.no_fp64)
+               if args.workload == "uneven" else X.fcd_stress())
         print(json.dumps(res), flush=True)
         return
 

@@ -32,7 +32,7 @@ EXPORTS = (
     "bnm_ctx_destroy", "bnm_ctx_set_sc", "bnm_ctx_set_noise", "bnm_ctx_set_empirical",
     "bnm_ctx_set_params", "bnm_ctx_run", "bnm_ctx_set_bold", "bnm_ctx_analyze", "bnm_ctx_get_hemi_fcd", "bnm_ctx_get_outputs", "bnm_ctx_timings",
     "bnm_ctx_kernel_name", "bnm_ctx_tune_log", "bnm_multi_create", "bnm_multi_run",
-    "bnm_multi_last_ms", "bnm_multi_destroy", "bnm_cmaes_create", "bnm_cmaes_destroy",
+    "bnm_multi_last_ms", "bnm_multi_plan", "bnm_multi_destroy", "bnm_cmaes_create", "bnm_cmaes_destroy",
     "bnm_cmaes_ask", "bnm_cmaes_tell", "bnm_cmaes_best", "bnm_cmaes_sigma", "bnm_cmaes_strategy", "bnm_cmaes_iteration", "bnm_cmaes_ask_pool", "bnm_cmaes_select",
     "bnm_mlp_create", "bnm_mlp_destroy", "bnm_mlp_num_params", "bnm_mlp_get_params",
     "bnm_mlp_set_params", "bnm_mlp_predict", "bnm_mlp_loss_grad", "bnm_mlp_set_data",
@@ -112,6 +112,8 @@ def lib():
     L.bnm_multi_run.argtypes = [vp]
     L.bnm_multi_last_ms.argtypes = [vp]
     L.bnm_multi_last_ms.restype = C.c_double
+    L.bnm_multi_plan.argtypes = [vp]
+    L.bnm_multi_plan.restype = C.c_char_p
     L.bnm_multi_destroy.argtypes = [vp]
     L.bnm_multi_destroy.restype = None
     L.bnm_cmaes_create.argtypes = [C.c_int, C.c_int, C.c_uint, C.c_double, C.c_int]
@@ -484,7 +486,8 @@ class MLP:
 
 
 class Multi:
-    """Several contexts run as parallel branches of one captured hipGraph."""
+    """Several contexts packed into branches of one captured hipGraph
+    (concurrent branches, contexts in sequence within a branch)."""
 
     def __init__(self, ctxs):
         self.ctxs = list(ctxs)
@@ -498,6 +501,9 @@ class Multi:
     def run(self):
         _chk(lib().bnm_multi_run(self.h))
         return lib().bnm_multi_last_ms(self.h)
+
+    def plan(self):
+        return lib().bnm_multi_plan(self.h).decode()
 
     def close(self):
         if getattr(self, "h", None):

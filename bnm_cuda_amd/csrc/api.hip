@@ -113,6 +113,7 @@ struct bnm_ctx {
     std::string kname = "none";
     std::string tune_log;
     bool tuned = false;
+    double ms_per_sim_ms = -1;   // integration time per simulated ms of the chosen variant (tuner)
     int autotune = 1, tune_ms = 300;
     Variant variant{0, 4, 1, 0};
     hipGraphExec_t graph_exec = nullptr;
@@ -781,6 +782,7 @@ static int choose_variant(bnm_ctx* x) {
             x->tune_log += buf;
             if (mn < best) { best = mn; x->variant = v; }
         }
+        x->ms_per_sim_ms = best / std::max(1, (int)a.time_steps + 1);
     }
     x->kname = variant_name(x, x->variant);
     x->tuned = true;
@@ -1006,7 +1008,33 @@ struct bnm_multi {
     hipEvent_t fork = nullptr, t0 = nullptr, t1 = nullptr;
     std::vector<hipEvent_t> joins;
     double ms = 0;
+    std::string plan;
 };
+
+// Integration time estimate of a tuned context (ms): the tuner's time per
+// simulated ms, or one timed window of the chosen variant when the tuner did
+// not time it, times the run's simulated ms (FIC restarts not included).
+static int estimate_ms(bnm_ctx* x, double* out) {
+    if (x->ms_per_sim_ms < 0) {
+        SimArgs a = make_args(x);
+        a.time_steps = std::min(x->cfg.time_steps, x->tune_ms);
+        a.adjust_fic = 0;
+        a.wIE_out = x->wIEscratch;
+        float ms = 0;
+        HIPCHK(hipEventRecord(x->ev[0], x->st));
+        HIPCHK(launch_variant(x, x->variant, a, x->st));
+        HIPCHK(hipEventRecord(x->ev[1], x->st));
+        HIPCHK(hipEventSynchronize(x->ev[1]));
+        HIPCHK(hipEventElapsedTime(&ms, x->ev[0], x->ev[1]));
+        x->ms_per_sim_ms = ms / std::max(1, (int)a.time_steps + 1);
+    }
+    *out = x->ms_per_sim_ms * (x->cfg.time_steps + 1);
+    return BNM_OK;
+}
+
+// workgroups of the context's integration kernel (one per simulation, or one
+// per 16 for the MFMA kernel)
+static int integration_wgs(const bnm_ctx* x) { return x->variant.kind == 2 ? (x->B + 15) / 16 : x->B; }
 
 bnm_multi* bnm_multi_create(bnm_ctx** ctxs, int n, int* status) {
     auto set = [&](int s) { if (status) *status = s; };
@@ -1030,19 +1058,71 @@ bnm_multi* bnm_multi_create(bnm_ctx** ctxs, int n, int* status) {
     for (auto& e : m->joins)
         if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
             return bad(fail(BNM_E_HIP, "hipEventCreate failed"));
-    // capture: the first context's stream forks to every other context's
-    // stream, each enqueues its run, and all join back
+    // Packing: the integration kernels hold one CU each per workgroup for
+    // the whole run (register and LDS budgets leave no room for a second),
+    // so populations started together beyond the CU count share CUs and slow
+    // the longest one down.  K = CUs / (largest workgroup count) branches run
+    // concurrently; contexts go to branches longest first, each to the branch
+    // with the least estimated work (LPT), and run one after another within
+    // it.  BNM_MULTI_BRANCHES overrides K (n: every context its own branch).
+    int cus = 0, dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+        cus = 256;
+    int max_wg = 1;
+    std::vector<double> est(n);
+    for (int i = 0; i < n; ++i) {
+        max_wg = std::max(max_wg, integration_wgs(m->ctxs[i]));
+        const int re = estimate_ms(m->ctxs[i], &est[i]);
+        if (re) return bad(re);
+    }
+    int K = env_int("BNM_MULTI_BRANCHES", 0);
+    if (K <= 0) K = std::max(1, cus / max_wg);
+    K = std::min(K, n);
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return est[a] > est[b]; });
+    std::vector<std::vector<int>> branch(K);
+    std::vector<double> load(K, 0.0);
+    for (int i : order) {
+        int b = 0;
+        for (int q = 1; q < K; ++q)
+            if (load[q] < load[b]) b = q;
+        branch[b].push_back(i);
+        load[b] += est[i];
+    }
+    {
+        char buf[96];
+        for (int b = 0; b < K; ++b) {
+            std::snprintf(buf, sizeof buf, "%sbranch %d (est %.0f ms):", b ? "; " : "", b, load[b]);
+            m->plan += buf;
+            for (int i : branch[b]) {
+                std::snprintf(buf, sizeof buf, " ctx %d (N=%d, %d wg, est %.0f ms)", i, m->ctxs[i]->N,
+                              integration_wgs(m->ctxs[i]), est[i]);
+                m->plan += buf;
+            }
+        }
+    }
+    // capture: the first context's stream forks to one stream per branch
+    // (its first context's), each branch enqueues its contexts in order, and
+    // all join back
     hipStream_t s0 = m->ctxs[0]->st;
     hipGraph_t gph = nullptr;
     if (hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal) != hipSuccess)
         return bad(fail(BNM_E_HIP, "hipStreamBeginCapture failed"));
     int r = BNM_OK;
     hipError_t e = hipEventRecord(m->fork, s0);
-    for (int i = 1; i < n && e == hipSuccess; ++i) e = hipStreamWaitEvent(m->ctxs[i]->st, m->fork, 0);
-    for (int i = 0; i < n && e == hipSuccess && !r; ++i) r = enqueue_run(m->ctxs[i], m->ctxs[i]->st, false);
-    for (int i = 1; i < n && e == hipSuccess && !r; ++i) {
-        e = hipEventRecord(m->joins[i], m->ctxs[i]->st);
-        if (e == hipSuccess) e = hipStreamWaitEvent(s0, m->joins[i], 0);
+    std::vector<hipStream_t> bst(K);
+    for (int b = 0; b < K; ++b) bst[b] = m->ctxs[branch[b][0]]->st;
+    for (int b = 0; b < K && e == hipSuccess; ++b)
+        if (bst[b] != s0) e = hipStreamWaitEvent(bst[b], m->fork, 0);
+    for (int b = 0; b < K && e == hipSuccess && !r; ++b)
+        for (int i : branch[b])
+            if (!r) r = enqueue_run(m->ctxs[i], bst[b], false);
+    for (int b = 0; b < K && e == hipSuccess && !r; ++b) {
+        if (bst[b] == s0) continue;
+        e = hipEventRecord(m->joins[b], bst[b]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s0, m->joins[b], 0);
     }
     const hipError_t e2 = hipStreamEndCapture(s0, &gph);
     if (r || e != hipSuccess || e2 != hipSuccess) {
@@ -1070,6 +1150,8 @@ int bnm_multi_run(bnm_multi* m) {
 }
 
 double bnm_multi_last_ms(bnm_multi* m) { return m ? m->ms : -1; }
+
+const char* bnm_multi_plan(bnm_multi* m) { return m ? m->plan.c_str() : ""; }
 
 void bnm_multi_destroy(bnm_multi* m) {
     if (!m) return;

@@ -4,8 +4,10 @@ printing one JSON object.
 uneven_batch  four populations of different node counts and durations (N =
               68/100/200/400, 128 simulations each, synthetic SCs of 20-40 %
               density, regional wEE_i / wEI_i = base + 0.1 z_i, 900/700/500/300 s)
-              run as branches of ONE captured hipGraph launch (FP32, autotuned
-              kernels), against the same populations run one after another, with
+              run in ONE captured hipGraph launch (FP32, autotuned kernels;
+              branches packed to the CU count, and for comparison every
+              population in its own branch), against the same populations run
+              one after another, with
               errors against FP64 runs and a BF16x3-coupling variant (MFMA kernel,
               N <= 128 populations).
 fcd_stress    N = 400, 16 simulations, 1201 volumes at TR 0.72 s, FCD window 44
@@ -82,7 +84,8 @@ def uneven_batch(B=128, durations_scale=1.0, fp64=True, bf16x3=True):
     ctxs = [ctx(p, _lib.PREC_F32) for p in pops]
     t0 = time.perf_counter()
     m = _lib.Multi(ctxs)
-    _log(f"[uneven] graph captured in {time.perf_counter() - t0:.1f} s (incl. tuning)")
+    plan = m.plan()
+    _log(f"[uneven] graph captured in {time.perf_counter() - t0:.1f} s (incl. tuning); {plan}")
     t0 = time.perf_counter()
     dev_ms = m.run()
     wall_graph = time.perf_counter() - t0
@@ -90,6 +93,17 @@ def uneven_batch(B=128, durations_scale=1.0, fp64=True, bf16x3=True):
     kernels = [c.kernel_name() for c in ctxs]
     _log(f"[uneven] graph: {wall_graph:.2f} s wall, {dev_ms:.0f} ms device, kernels {kernels}")
     m.close()
+    # the same graph with every population in its own concurrent branch
+    os.environ["BNM_MULTI_BRANCHES"] = str(len(ctxs))
+    try:
+        m = _lib.Multi(ctxs)
+    finally:
+        os.environ.pop("BNM_MULTI_BRANCHES")
+    t0 = time.perf_counter()
+    m.run()
+    wall_all = time.perf_counter() - t0
+    m.close()
+    _log(f"[uneven] every population its own branch: {wall_all:.2f} s wall")
     # the same populations one after another (eager, tuned variants reused)
     t0 = time.perf_counter()
     seq_each = []
@@ -109,7 +123,8 @@ def uneven_batch(B=128, durations_scale=1.0, fp64=True, bf16x3=True):
         "config": {"workload": "4 populations x %d sims: N/duration %s, regional wEE/wEI = base + 0.1 z, "
                                "FIC on, FCD window 30 step 5" % (B, [(p["N"], p["T"] // 1000) for p in pops]),
                    "nominal_sim_s": nominal, "kernels": kernels, "fic_trials": trials},
-        "graph_wall_s": round(wall_graph, 3), "graph_device_ms": round(dev_ms, 1),
+        "graph_wall_s": round(wall_graph, 3), "graph_device_ms": round(dev_ms, 1), "graph_plan": plan,
+        "graph_all_concurrent_wall_s": round(wall_all, 3),
         "sequential_wall_s": round(wall_seq, 3), "sequential_each_s": seq_each,
         "graph_vs_sequential": round(wall_seq / wall_graph, 2),
     }

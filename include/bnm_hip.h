@@ -220,15 +220,19 @@ const char* bnm_ctx_kernel_name(bnm_ctx* ctx);
  * did not run). */
 const char* bnm_ctx_tune_log(bnm_ctx* ctx);
 
-/* ---- several populations in one graph launch: each context (its own SC,
- * node count, duration and parameters) becomes a parallel branch of one
- * captured hipGraph; bnm_multi_run launches it once and waits.  Contexts
- * must have their inputs set; outputs are read per context afterwards.
- * No counterpart in the reference (one population per call). ---- */
+/* ---- several populations in one graph launch: the contexts (each with its
+ * own SC, node count, duration and parameters) are packed into branches of
+ * one captured hipGraph -- as many concurrent branches as the CUs hold
+ * integration workgroups, longest populations first, each branch running its
+ * contexts one after another -- and bnm_multi_run launches it once and waits.
+ * Contexts must have their inputs set; outputs are read per context
+ * afterwards.  No counterpart in the reference (one population per call). ---- */
 typedef struct bnm_multi bnm_multi;
 bnm_multi* bnm_multi_create(bnm_ctx** ctxs, int n, int* status);
 int bnm_multi_run(bnm_multi* m);
 double bnm_multi_last_ms(bnm_multi* m);
+/* The branch plan (contexts per branch, workgroups, time estimates). */
+const char* bnm_multi_plan(bnm_multi* m);
 void bnm_multi_destroy(bnm_multi* m);
 
 /* ---- CMA-ES over the unit box (cmaes.cpp of the reference: CMAES_Points,

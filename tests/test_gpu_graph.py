@@ -83,9 +83,14 @@ def test_graph_replay_matches_eager(precision):
     eager.close()
 
 
-def test_uneven_populations_in_one_graph_launch():
-    """Three populations with N = 20, 34, 68 and durations 60, 80, 70 s run as
-    branches of one graph; each matches its own eager run bitwise."""
+@pytest.mark.parametrize("branches", [None, "2", "1"])
+def test_uneven_populations_in_one_graph_launch(monkeypatch, branches):
+    """Three populations with N = 20, 34, 68 and durations 60, 80, 70 s in one
+    graph; each matches its own eager run bitwise.  Default packing puts each
+    in its own branch (their workgroups fit the CUs together); two branches
+    chain two populations in one; one branch runs all three in sequence."""
+    if branches is not None:
+        monkeypatch.setenv("BNM_MULTI_BRANCHES", branches)
     specs = [(20, 6, 60_000, 3), (34, 5, 80_000, 4), (68, 17, 70_000, 5)]
     ctxs, refs = [], []
     for N, B, T, seed in specs:
@@ -97,6 +102,9 @@ def test_uneven_populations_in_one_graph_launch():
         e.close()
         ctxs.append(_ctx(*pop))
     m = bnm.Multi(ctxs)
+    plan = m.plan()
+    print(plan)
+    assert plan.count("branch") == (3 if branches is None else int(branches))
     for rep in range(2):
         ms = m.run()
         assert ms > 0
