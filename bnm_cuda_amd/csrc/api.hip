@@ -63,6 +63,7 @@ struct Variant {
                 // 4 split with the E/I pools on separate lanes (FP32; mode = form:
                 //   0 as written, 1 S_E by DPP row broadcast, 2 / 3 short-chain
                 //   update with four / one partial sums),
+                // 6 FP64 split with the E/I pools on separate lanes,
                 // 5 stream with register/LDS-resident SC (FP32; mode 1: S_E by
                 //   DPP row broadcast, 2: and three streamed chunks in flight)
     int ls, ks, mode;
@@ -603,6 +604,7 @@ static bool variant_ok(const bnm_ctx* x, const Variant& v) {
         case 2: return !x->f64 && mfma_shape(x->N, v.mode).ok;
         case 4: return !x->f64 && ei_shape(x->N, v.ls, v.mode == 1).ok;
         case 5: return !x->f64 && hyb_shape(x->N, v.ks, x->lds_max, x->hyb_rr, v.mode).ok;
+        case 6: return x->f64 && ei64_shape(x->N, v.ls).ok;
         default: return true;
     }
 }
@@ -629,6 +631,10 @@ static std::string variant_name(const bnm_ctx* x, const Variant& v) {
         std::snprintf(name, sizeof name, "hyb_f32(KS=%d,NW=%d,kc=%d,regs=%d,lds=%d,streamed=%d,threads=%d%s)", g.KS,
                       g.NW, g.kc, g.KC, g.SS, g.kc - g.KC - g.SS, g.threads,
                       v.mode == 2 ? ",dpp,pf3" : v.mode == 1 ? ",dpp" : "");
+    } else if (v.kind == 6) {
+        const FastShape s = ei64_shape(x->N, v.ls);
+        std::snprintf(name, sizeof name, "split_ei_f64(LS=%d,NW=%d,KC=%d,SS=%d,threads=%d)", s.LS, s.NW, s.KC,
+                      s.SS, s.threads);
     } else if (v.kind == 4) {
         const FastShape s = ei_shape(x->N, v.ls, v.mode == 1);
         static const char* fm[] = {"", ",dpp", ",short4", ",short1"};
@@ -651,6 +657,7 @@ static hipError_t launch_variant(bnm_ctx* x, const Variant& v, const SimArgs& a,
         case 2: return launch_sim_mfma(x->SC32, a, v.mode, st);
         case 4: return launch_sim_ei(x->SC32, a, v.ls, v.mode, st);
         case 5: return launch_sim_hyb(x->SCc32, a, v.ks, x->lds_max, x->hyb_rr, v.mode, st);
+        case 6: return launch_sim_ei64(x->SC64, a, v.ls, st);
         default:
             return x->f64 ? launch_sim_plain<double>(x->SC64, a, st)
                           : launch_sim_plain<float>(x->SC32, a, st);
@@ -679,8 +686,9 @@ static std::vector<Variant> candidates(const bnm_ctx* x) {
         add({5, 1, x->hyb_ks, x->hyb_bc > 0 ? x->hyb_bc : 0});
         if (!out.empty()) return out;
     }
-    if (x->ei_ls > 0 && !x->f64) {
-        add({4, x->ei_ls, 1, x->ei_form >= 0 ? x->ei_form : x->ei_bc == 1 ? 1 : 0});
+    if (x->ei_ls > 0) {
+        if (x->f64) add({6, x->ei_ls, 1, 0});
+        else add({4, x->ei_ls, 1, x->ei_form >= 0 ? x->ei_form : x->ei_bc == 1 ? 1 : 0});
         if (!out.empty()) return out;
     }
     const bool pinned = x->fast_ls > 0 || x->fast_ks > 0;
@@ -689,6 +697,10 @@ static std::vector<Variant> candidates(const bnm_ctx* x) {
         if (!out.empty()) return out;
     }
     if (!x->force_stream) {
+        if (x->f64 && !x->no_ei) {
+            add({6, 2, 1, 0});
+            add({6, 4, 1, 0});
+        }
         if (!x->f64 && !x->no_ei) {
             if (x->ei_bc == 1) add({4, 4, 1, 1});
             // forms 3 and 2 (short chain) are tuned by default; form 0 only when pinned

@@ -41,6 +41,10 @@ hipError_t launch_sim_hyb(const float* SCc, const SimArgs& a, int KS, size_t lds
                           hipStream_t st);
 // FP32 split kernel with the node update divided between lanes by population
 FastShape ei_shape(int N, int LS, bool bc = false);
+// FP64 E/I split (reference expression order per pool; bit-identical to the
+// FP64 split kernel with the same LS)
+FastShape ei64_shape(int N, int LS);
+hipError_t launch_sim_ei64(const double* SC, const SimArgs& a, int LS, hipStream_t st);
 // form: 0 as written, 1 DPP row broadcast (LS = 4, N <= 128), 2 / 3 short-chain
 // update with four / one partial sums in the coupling dot
 hipError_t launch_sim_ei(const float* SC, const SimArgs& a, int LS, int form, hipStream_t st);

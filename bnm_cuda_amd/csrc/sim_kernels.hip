@@ -11,6 +11,7 @@
 //                  (see the kernel's comment); one barrier per substep when
 //                  KS == 1.
 #include <algorithm>
+#include <type_traits>
 
 #include "kernels.hpp"
 #include "model.hpp"
@@ -617,6 +618,183 @@ __global__ __launch_bounds__(MAXT) void k_sim_ei(const float* __restrict__ SC, S
 }
 
 // ---------------------------------------------------------------------------
+// split_ei (FP64): the E/I lane division of the FP32 kernel above with the
+// reference's expression order per pool (bnm.cpp:181-200, contraction off):
+//   I  = ((k0 + kSE*S_E) + (c*Gl)*J_NMDA) - kSI*S_I   E: wE_I0, wEE, G, wIE
+//                                                     I: wI_I0, wEI, 0, 1
+//   x  = a*I - b,  r = x / (1 - exp(-d*x))
+//   dS = (xi*sigma)*sqrt_dt + dt*(((1 - m*S)*gamma)*r - S/tau)   m = 1 (E), 0 (I)
+// For the inhibitory lanes the extra terms are exact: (c*0)*J = +0 (c >= 0),
+// 1*S_I = S_I and (1 - 0*S)*gamma = gamma, so each pool gets the reference's
+// own expression.  The coupling dot and its reduction are those of the FP64
+// split kernel with the same LS, so the results are bit-identical to it; each
+// lane evaluates one FP64 exp per substep instead of two.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void ei_step_f64(double c, double Gl, double k0, double kSE, double kSI, double aa,
+                                            double bb, double dd, double g, double m, double tau, double xi,
+                                            double SEv, double SIv, double& S, double& I, double& r) {
+#pragma clang fp contract(off)
+    I = k0 + kSE * SEv + c * Gl * C64::J_NMDA - kSI * SIv;
+    const double x = aa * I - bb;
+    const double e = exp(-dd * x);
+    r = x / (1 - e);
+    const double dS = xi * C64::sigma * C64::sqrt_dt + C64::dt * ((1 - m * S) * g * r - (S / tau));
+    S = fmax(0.0, fmin(S + dS, 1.0));
+}
+
+template <int KC, int LS, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_sim_ei64(const double* __restrict__ SC, SimArgs a, int NW, int SS) {
+    static_assert(LS == 2 || LS == 4, "E/I split needs two lanes per node");
+    static_assert(KC % 4 == 0, "slice length must be a multiple of 4");
+    constexpr int GN = 64 / LS;
+    const int sim = blockIdx.x;
+    const int N = a.N;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ln = lane % GN, sl = lane / GN;
+    const int j = wave * GN + ln;
+    const int pop = sl & 1;               // 0: excitatory pool, 1: inhibitory
+    const int k0i = sl * KC;
+    const bool comp = j < N;
+    const bool writer = comp && sl == 0;  // publishes S_E, BOLD, wIE
+    const bool owner = comp && sl < 2;    // writes its pool's means
+    const int ldS = LS * SS;
+    const int posj = (j / KC) * SS + (j % KC);
+    extern __shared__ unsigned char smem_raw[];
+    double* Sb = reinterpret_cast<double*>(smem_raw);   // [2][ldS]
+
+    double sc[KC];
+#pragma unroll
+    for (int q = 0; q < KC; ++q) {
+        const int k = k0i + q;
+        sc[q] = (j < N && k < N) ? SC[(size_t)j * N + k] : 0.0;
+    }
+    for (int i = threadIdx.x; i < 2 * ldS; i += blockDim.x) {
+        const int slot = i % ldS, qq = slot % SS, k = (slot / SS) * KC + qq;
+        Sb[i] = (qq < KC && k < N) ? 0.001 : 0.0;
+    }
+    const size_t o = (size_t)sim * N + (comp ? j : 0);
+    const double Gl = pop ? 0.0 : a.G[sim];
+    const double k0 = pop ? C64::wI_I0 : C64::wE_I0;
+    const double kSE = comp ? (pop ? a.wEI[o] : a.wEE[o]) : 0.0;
+    const double aa = pop ? C64::a_I : C64::a_E, bb = pop ? C64::b_I : C64::b_E, dd = pop ? C64::d_I : C64::d_E;
+    const double g = pop ? C64::gamma_I : C64::gamma_E, m = pop ? 0.0 : 1.0, tau = pop ? C64::tau_I : C64::tau_E;
+    double wIE = comp ? a.wIE_in[o] : 0.0, delta = a.init_delta;
+    double S = 0.001, I = 0.0, r = 0.0;
+    BW<double> bw;
+    double mI = 0, ext[3] = {0, 0, 0};
+
+    int ts = 0, vol = 0, trial = 0, buf = 0;
+    bool adjusting = a.adjust_fic != 0, failed = false;
+    const unsigned dur = (unsigned)(a.i_end - a.i_start + 1);
+    const int P = (int)a.noise_period;
+    const float* noise1 = a.noise + 2 * (size_t)(comp ? j : 0) + pop;   // this pool's draw
+    float nz[10], nzn[10];
+    int ms = 0;
+#pragma unroll
+    for (int s = 0; s < 10; ++s) nz[s] = comp ? noise1[(size_t)s * 2 * N] : 0.0f;
+    __syncthreads();
+    while (ts <= a.time_steps) {
+        const int ms_next = ms + 1 == P ? 0 : ms + 1;
+        if (comp) {
+            const float* p = noise1 + (size_t)ms_next * 20 * N;
+#pragma unroll
+            for (int s = 0; s < 10; ++s) nzn[s] = p[(size_t)s * 2 * N];
+        }
+        const double kSI = pop ? 1.0 : wIE;
+#pragma unroll
+        for (int s = 0; s < 10; ++s) {
+            const double2* Sr = reinterpret_cast<const double2*>(Sb + buf * ldS + sl * SS);
+            double2 sv[KC / 2];
+#pragma unroll
+            for (int q = 0; q < KC / 2; ++q) sv[q] = Sr[q];
+            // previous-substep S of both pools: the lower row holds S_E, the
+            // upper S_I (slices 2 and 3 mirror 0 and 1)
+            double SEv, SIv;
+            {
+                const unsigned long long u = __double_as_longlong(S);
+                const unsigned lo = (unsigned)u, hi = (unsigned)(u >> 32);
+                auto rl = LS == 4 ? __builtin_amdgcn_permlane16_swap(lo, lo, false, false)
+                                  : __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+                auto rh = LS == 4 ? __builtin_amdgcn_permlane16_swap(hi, hi, false, false)
+                                  : __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+                SEv = __longlong_as_double(((unsigned long long)rh[0] << 32) | rl[0]);
+                SIv = __longlong_as_double(((unsigned long long)rh[1] << 32) | rl[1]);
+            }
+            double acc[4] = {0, 0, 0, 0};
+#pragma unroll
+            for (int q = 0; q < KC / 2; ++q) {
+                acc[(2 * q) & 3] = fma(sc[2 * q + 0], sv[q].x, acc[(2 * q) & 3]);
+                acc[(2 * q + 1) & 3] = fma(sc[2 * q + 1], sv[q].y, acc[(2 * q + 1) & 3]);
+            }
+            double c = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+            if constexpr (LS == 4) c = lane_sum16<double>(c);
+            c = lane_sum32<double>(c);
+            ei_step_f64(c, Gl, k0, kSE, kSI, aa, bb, dd, g, m, tau, (double)nz[s], SEv, SIv, S, I, r);
+            if (writer) Sb[(buf ^ 1) * ldS + posj] = S;
+            __syncthreads();
+            buf ^= 1;
+        }
+        if (comp && pop == 0) bw.step(S);
+        if (ts % a.bold_tr == 0) {
+            if (writer) {
+                double* bold = reinterpret_cast<double*>(a.bold);
+                bold[((size_t)sim * a.output_ts + vol) * N + j] = bw.bold();
+            }
+            if (vol >= a.n_vols_remove) {
+                ext[0] += S;
+                ext[1] += I;
+                ext[2] += r;
+            }
+            ++vol;
+        }
+        ++ts;
+        bool restarted = false;
+        if (adjusting) {
+            if (ts >= a.i_start && ts <= a.i_end) mI += I;
+            if (ts == a.i_end) {
+                // the excitatory lanes of a node decide alike; inhibitory lanes never do
+                const bool out = comp && pop == 0 && fic_decide<double>(mI, dur, trial, a.max_fic_trials, wIE, delta);
+                if (__syncthreads_or(out)) {
+                    if (trial < a.max_fic_trials) {
+                        S = 0.001;
+                        bw.reset();
+                        mI = 0;
+                        ext[0] = ext[1] = ext[2] = 0;
+                        if (writer) Sb[buf * ldS + posj] = 0.001;
+                        ts = 0; vol = 0; ++trial;
+                        restarted = true;
+                        __syncthreads();
+                    } else {
+                        failed = true; adjusting = false;
+                    }
+                } else {
+                    adjusting = false;
+                }
+            }
+        }
+        if (restarted) {
+            ms = 0;
+#pragma unroll
+            for (int s = 0; s < 10; ++s) nz[s] = comp ? noise1[(size_t)s * 2 * N] : 0.0f;
+        } else {
+            ms = ms_next;
+#pragma unroll
+            for (int s = 0; s < 10; ++s) nz[s] = nzn[s];
+        }
+    }
+    if (owner) {
+        if (pop == 0) a.wIE_out[o] = wIE;
+        const int n = vol - a.n_vols_remove;
+        for (int i = 0; i < 3; ++i)
+            a.ext[((size_t)sim * 6 + 3 * pop + i) * N + j] = n > 0 ? ext[i] / n : 0.0;
+    }
+    if (threadIdx.x == 0) {
+        a.fic_failed[sim] = failed ? 1 : 0;
+        a.fic_trials[sim] = trial;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // split_g: the split kernel for node counts whose SC does not fit in
 // registers (N > 256).  Lane j streams its SC column slice from a transposed
 // copy SCt[k][j] in global memory (L2-resident; consecutive lanes read
@@ -1090,6 +1268,38 @@ hipError_t launch_sim_ei(const float* SC, const SimArgs& a, int LS, int form, hi
         case 64: launch_ei_kc<64>(SC, a, s, form, st); break;
         case 100: launch_ei_kc<100>(SC, a, s, form, st); break;
         default: launch_ei_kc<128>(SC, a, s, form, st); break;
+    }
+    return hipGetLastError();
+}
+
+template <int KC, int LS>
+static void launch_ei64(const double* SC, const SimArgs& a, const FastShape& s, hipStream_t st) {
+    constexpr int MAXT = maxt_for(KC, 8);
+    hipLaunchKernelGGL((k_sim_ei64<KC, LS, MAXT>), dim3(a.B), dim3(s.threads), s.smem, st, SC, a, s.NW, s.SS);
+}
+
+FastShape ei64_shape(int N, int LS) {
+    FastShape s = fast_shape(N, LS, 1, sizeof(double));
+    s.ok = s.ok && (LS == 2 || LS == 4);
+    return s;
+}
+
+hipError_t launch_sim_ei64(const double* SC, const SimArgs& a, int LS, hipStream_t st) {
+    const FastShape s = ei64_shape(a.N, LS);
+    if (!s.ok) return hipErrorInvalidValue;
+    auto go = [&](auto kc) {
+        constexpr int KC = decltype(kc)::value;
+        if (s.LS == 2) launch_ei64<KC, 2>(SC, a, s, st);
+        else launch_ei64<KC, 4>(SC, a, s, st);
+    };
+    switch (s.KC) {
+        case 8: go(std::integral_constant<int, 8>{}); break;
+        case 16: go(std::integral_constant<int, 16>{}); break;
+        case 28: go(std::integral_constant<int, 28>{}); break;
+        case 32: go(std::integral_constant<int, 32>{}); break;
+        case 52: go(std::integral_constant<int, 52>{}); break;
+        case 64: go(std::integral_constant<int, 64>{}); break;
+        default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }

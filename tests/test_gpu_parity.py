@@ -128,6 +128,39 @@ def test_f64_split_kernel_matches_oracle(case, ref, split):
     np.testing.assert_allclose(g["corr"], ref["corr"], atol=1e-9)
 
 
+def _ctx_run(case, cfg):
+    with bnm.Context(cfg) as c:
+        c.set_sc(case["SC"])
+        c.set_noise(case["noise"])
+        c.set_empirical(case["emp_fc"], case["emp_fcd"])
+        c.set_params(case["G"], case["wEE"], case["wEI"])
+        c.run()
+        return c.kernel_name(), c.outputs()
+
+
+@pytest.mark.parametrize("ls", [2, 4])
+def test_f64_ei_split_bit_identical_to_split_and_oracle(case, ref, ls, monkeypatch):
+    """FP64 kernel with the E and I pools on separate lanes, each pool in the
+    reference's expression order: bit-identical to the FP64 split kernel with
+    the same lane split (same coupling dot and reduction), hence within the
+    FP64 bars of the oracle."""
+    print("case: f64 ei split LS", ls, flush=True)
+    monkeypatch.setenv("BNM_EI_LS", str(ls))
+    ke, e = _ctx_run(case, _cfg(case, bnm.PREC_F64, bnm.KERNEL_FAST))
+    monkeypatch.delenv("BNM_EI_LS")
+    monkeypatch.setenv("BNM_FAST_LS", str(ls))
+    monkeypatch.setenv("BNM_FAST_KS", "1")
+    ks, sp = _ctx_run(case, _cfg(case, bnm.PREC_F64, bnm.KERNEL_FAST))
+    print(ke, "|", ks)
+    assert ke.startswith("split_ei_f64(LS=%d" % ls) and ks.startswith("split_f64(LS=%d" % ls)
+    for k in ("bold", "ext", "wIE_fic", "fic_trials", "fic_failed", "fc", "fcd", "corr", "fic_penalty"):
+        np.testing.assert_array_equal(e[k], sp[k], err_msg=k)
+    np.testing.assert_array_equal(e["fic_trials"], ref["fic_trials"])
+    assert _bold_rel(e["bold"], ref["bold"]) <= 1e-6
+    np.testing.assert_allclose(e["ext"], ref["ext"], rtol=1e-7, atol=1e-9)
+    np.testing.assert_allclose(e["corr"], ref["corr"], atol=1e-9)
+
+
 @pytest.mark.parametrize("kernel", [bnm.KERNEL_PLAIN, bnm.KERNEL_FAST])
 def test_f64_bold_fic_fit_match_oracle(case, ref, kernel):
     g = _gpu(case, _cfg(case, bnm.PREC_F64, kernel))
