@@ -64,6 +64,7 @@ struct Variant {
                 //   0 as written, 1 S_E by DPP row broadcast, 2 / 3 short-chain
                 //   update with four / one partial sums),
                 // 6 FP64 split with the E/I pools on separate lanes,
+                // 7 FP64 stream with register/LDS-resident SC,
                 // 5 stream with register/LDS-resident SC (FP32; mode 1: S_E by
                 //   DPP row broadcast, 2: and three streamed chunks in flight)
     int ls, ks, mode;
@@ -85,6 +86,7 @@ struct bnm_ctx {
     double* SCt64 = nullptr;  // transposed copies for the streamed large-N kernel
     float* SCt32 = nullptr;
     float* SCc32 = nullptr;   // chunked [k/16][j][k%16] copy for the hybrid kernel
+    double* SCc64 = nullptr;  // chunked [k/8][j][k%8] copy for the FP64 hybrid kernel
     size_t lds_max = 64 * 1024;  // dynamic LDS a workgroup may use
     float* noise = nullptr;
     long long noise_cap = 0;
@@ -270,7 +272,7 @@ int bnm_make_noise_chunked(unsigned seed, long long count, float* out, int threa
 
 void bnm_ctx_destroy(bnm_ctx* x) {
     if (!x) return;
-    void* bufs[] = {x->SC64, x->SC32, x->SCt64, x->SCt32, x->SCc32, x->noise, x->G, x->wEE, x->wEI, x->wIE, x->wIEfic,
+    void* bufs[] = {x->SC64, x->SC32, x->SCt64, x->SCt32, x->SCc32, x->SCc64, x->noise, x->G, x->wEE, x->wEI, x->wIE, x->wIEfic,
                     x->wIEscratch, x->bold, x->ext, x->failed, x->trials, x->unstable,
                     x->seg_start, x->seg_len, x->pidx, x->mean, x->inv, x->wmean, x->winv, x->fc,
                     x->wfc, x->wz, x->wzlo, x->fcd, x->emp_c, x->ks_u, x->ks_fe, x->bins, x->nanf, x->corr, x->pen,
@@ -298,6 +300,10 @@ static int ctx_alloc(bnm_ctx* x) {
     HIPCHK(hipMemset(x->SCt32, 0, (N + kSCtPad) * N * sizeof(float)));
     if ((r = dalloc(&x->SCc32, (N + kSCtPad) * N))) return r;
     HIPCHK(hipMemset(x->SCc32, 0, (N + kSCtPad) * N * sizeof(float)));
+    if (x->f64) {
+        if ((r = dalloc(&x->SCc64, (N + kSCtPad) * N))) return r;
+        HIPCHK(hipMemset(x->SCc64, 0, (N + kSCtPad) * N * sizeof(double)));
+    }
     if ((r = dalloc(&x->G, B))) return r;
     if ((r = dalloc(&x->wEE, B * N))) return r;
     if ((r = dalloc(&x->wEI, B * N))) return r;
@@ -515,6 +521,13 @@ int bnm_ctx_set_sc(bnm_ctx* x, const double* SC) {
     for (size_t j = 0; j < N; ++j)
         for (size_t k = 0; k < N; ++k) fc[((k / 16) * N + j) * 16 + k % 16] = (float)SC[j * N + k];
     HIPCHK(hipMemcpy(x->SCc32, fc.data(), fc.size() * sizeof(float), hipMemcpyHostToDevice));
+    if (x->SCc64) {
+        // SCc64[((k/8)*N + j)*8 + k%8] = SC[j][k]
+        std::vector<double> dc(((N + 7) / 8) * 8 * N, 0.0);
+        for (size_t j = 0; j < N; ++j)
+            for (size_t k = 0; k < N; ++k) dc[((k / 8) * N + j) * 8 + k % 8] = SC[j * N + k];
+        HIPCHK(hipMemcpy(x->SCc64, dc.data(), dc.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
     x->have_sc = true;
     return BNM_OK;
 }
@@ -605,6 +618,7 @@ static bool variant_ok(const bnm_ctx* x, const Variant& v) {
         case 4: return !x->f64 && ei_shape(x->N, v.ls, v.mode == 1).ok;
         case 5: return !x->f64 && hyb_shape(x->N, v.ks, x->lds_max, x->hyb_rr, v.mode).ok;
         case 6: return x->f64 && ei64_shape(x->N, v.ls).ok;
+        case 7: return x->f64 && hyb64_shape(x->N, v.ks, x->lds_max, x->hyb_rr).ok;
         default: return true;
     }
 }
@@ -631,6 +645,10 @@ static std::string variant_name(const bnm_ctx* x, const Variant& v) {
         std::snprintf(name, sizeof name, "hyb_f32(KS=%d,NW=%d,kc=%d,regs=%d,lds=%d,streamed=%d,threads=%d%s)", g.KS,
                       g.NW, g.kc, g.KC, g.SS, g.kc - g.KC - g.SS, g.threads,
                       v.mode == 2 ? ",dpp,pf3" : v.mode == 1 ? ",dpp" : "");
+    } else if (v.kind == 7) {
+        const FastShape g = hyb64_shape(x->N, v.ks, x->lds_max, x->hyb_rr);
+        std::snprintf(name, sizeof name, "hyb_f64(KS=%d,NW=%d,kc=%d,regs=%d,lds=%d,streamed=%d,threads=%d)", g.KS,
+                      g.NW, g.kc, g.KC, g.SS, g.kc - g.KC - g.SS, g.threads);
     } else if (v.kind == 6) {
         const FastShape s = ei64_shape(x->N, v.ls);
         std::snprintf(name, sizeof name, "split_ei_f64(LS=%d,NW=%d,KC=%d,SS=%d,threads=%d)", s.LS, s.NW, s.KC,
@@ -658,6 +676,7 @@ static hipError_t launch_variant(bnm_ctx* x, const Variant& v, const SimArgs& a,
         case 4: return launch_sim_ei(x->SC32, a, v.ls, v.mode, st);
         case 5: return launch_sim_hyb(x->SCc32, a, v.ks, x->lds_max, x->hyb_rr, v.mode, st);
         case 6: return launch_sim_ei64(x->SC64, a, v.ls, st);
+        case 7: return launch_sim_hyb64(x->SCc64, a, v.ks, x->lds_max, x->hyb_rr, st);
         default:
             return x->f64 ? launch_sim_plain<double>(x->SC64, a, st)
                           : launch_sim_plain<float>(x->SC32, a, st);
@@ -682,8 +701,9 @@ static std::vector<Variant> candidates(const bnm_ctx* x) {
         add({2, 0, 0, 0});
         return out;
     }
-    if (x->hyb_ks > 0 && !x->f64) {
-        add({5, 1, x->hyb_ks, x->hyb_bc > 0 ? x->hyb_bc : 0});
+    if (x->hyb_ks > 0) {
+        if (x->f64) add({7, 1, x->hyb_ks, 0});
+        else add({5, 1, x->hyb_ks, x->hyb_bc > 0 ? x->hyb_bc : 0});
         if (!out.empty()) return out;
     }
     if (x->ei_ls > 0) {
@@ -717,11 +737,14 @@ static std::vector<Variant> candidates(const bnm_ctx* x) {
     if (out.empty() || x->force_stream) {
         if (x->stream_ks > 0) {
             add({1, 1, x->stream_ks, 0});
+            if (x->f64 && !x->no_hyb) add({7, 1, x->stream_ks, 0});
             if (!x->f64 && !x->no_hyb)
                 for (int m : {0, 1, 2})
                     if (x->hyb_bc < 0 || x->hyb_bc == m) add({5, 1, x->stream_ks, m});
         } else {
             for (int k : {16, 8, 4, 2, 1}) add({1, 1, k, 0});
+            if (x->f64 && !x->no_hyb)
+                for (int k : {8, 4, 2, 1}) add({7, 1, k, 0});
             if (!x->f64 && !x->no_hyb)
                 for (int k : {8, 4, 2, 1})
                     for (int m : {0, 1, 2})

@@ -41,6 +41,10 @@ hipError_t launch_sim_hyb(const float* SCc, const SimArgs& a, int KS, size_t lds
                           hipStream_t st);
 // FP32 split kernel with the node update divided between lanes by population
 FastShape ei_shape(int N, int LS, bool bc = false);
+// FP64 hybrid: register / LDS-resident SC, the rest streamed from the chunked
+// copy SCc64[((k/8)*N + j)*8 + k%8] (zero for k >= N, kSCtPad padding rows)
+FastShape hyb64_shape(int N, int KS, size_t lds_bytes, int rr = -1);
+hipError_t launch_sim_hyb64(const double* SCc, const SimArgs& a, int KS, size_t lds_bytes, int rr, hipStream_t st);
 // FP64 E/I split (reference expression order per pool; bit-identical to the
 // FP64 split kernel with the same LS)
 FastShape ei64_shape(int N, int LS);

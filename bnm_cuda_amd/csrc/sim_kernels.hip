@@ -795,6 +795,153 @@ __global__ __launch_bounds__(MAXT) void k_sim_ei64(const double* __restrict__ SC
 }
 
 // ---------------------------------------------------------------------------
+// hyb64 (FP64, N > 128): the hybrid kernel's residency scheme for the FP64
+// path.  Lane (j, ks) owns SC[j][k0 .. k0+kc); the first RR values stay in
+// VGPRs, the next LL in LDS ([kk/2][thread] double2 slots), the rest is
+// streamed from L2, 8 values (64 bytes, four 16-byte buffer loads) per chunk,
+// double-buffered, from a chunked copy SCc64[((k/8)*N + j)*8 + k%8].  S_E of
+// the previous substep is read from LDS by 16-byte broadcast loads; the node
+// update is the FP64 split kernel's (reference order, Node<double>).
+// ---------------------------------------------------------------------------
+template <int RR, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_sim_hyb64(const double* __restrict__ SCc, SimArgs a, int NW, int KS,
+                                                    int kc, int LL) {
+    constexpr int CH = 8;
+    static_assert(RR % CH == 0, "register part must be whole chunks");
+    const int sim = blockIdx.x;
+    const int N = a.N;
+    const int T = blockDim.x;
+    const int tid = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    const int nw = wave % NW, ks = wave / NW;
+    const int j = nw * 64 + lane;
+    const bool comp = ks == 0 && j < N;
+    const int k0 = ks * kc;
+    const int ldS = KS * kc;
+    const int jc = j < N ? j : N - 1;
+    extern __shared__ unsigned char smem_raw[];
+    double* Sb = reinterpret_cast<double*>(smem_raw);   // [2][ldS]
+    double* part = Sb + 2 * ldS;                          // [KS][NW*64]
+    double2* SCl = reinterpret_cast<double2*>(part + KS * NW * 64);   // [LL/2][T]
+    for (int i = tid; i < 2 * ldS; i += T) Sb[i] = (i % ldS) < N ? 0.001 : 0.0;
+    // chunks k/8 >= N/8 hold zeros (padding); the resource's size makes any
+    // read past the array return 0
+    const __amdgpu_buffer_rsrc_t scr = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<double*>(SCc), (short)0, (int)((N + (int)kSCtPad) * N * sizeof(double)), 0x00020000);
+    const int voff = jc * CH * (int)sizeof(double);
+    auto chunk = [&](int kb, double* w) {     // the lane's 8 values of chunk kb
+        const int so = kb * N * CH * (int)sizeof(double);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const auto v = __builtin_amdgcn_raw_buffer_load_b128(scr, voff + 16 * q, so, 0);
+            w[2 * q + 0] = __longlong_as_double(((long long)v[1] << 32) | v[0]);
+            w[2 * q + 1] = __longlong_as_double(((long long)v[3] << 32) | v[2]);
+        }
+    };
+    double wr[RR > 0 ? RR : 1];
+#pragma unroll
+    for (int c = 0; c < RR / CH; ++c) chunk(k0 / CH + c, wr + c * CH);
+    for (int c = 0; c < LL / CH; ++c) {
+        double w[CH];
+        chunk((k0 + RR) / CH + c, w);
+#pragma unroll
+        for (int q = 0; q < CH / 2; ++q) SCl[(size_t)(c * (CH / 2) + q) * T + tid] = double2{w[2 * q], w[2 * q + 1]};
+    }
+    const int gb = (k0 + RR + LL) / CH;   // first streamed chunk
+    const int nch = (kc - RR - LL) / CH;  // streamed chunks
+
+    Lane<double> L;
+    if (comp) load_params(a, sim, j, L);
+    const double gp = gain<double>(a.G[sim]);
+    int ts = 0, vol = 0, trial = 0, buf = 0;
+    bool adjusting = a.adjust_fic != 0, failed = false;
+    const unsigned dur = (unsigned)(a.i_end - a.i_start + 1);
+    const float2* noise2 = reinterpret_cast<const float2*>(a.noise);
+    __syncthreads();
+    while (ts <= a.time_steps) {
+        const long long ms = ts % a.noise_period;
+        const float2* nzp = noise2 + (size_t)ms * 10 * N + j;
+#pragma unroll 1
+        for (int s = 0; s < 10; ++s) {
+            const float2 nz = comp ? nzp[(size_t)s * N] : float2{0.0f, 0.0f};
+            double w0[CH], w1[CH];
+            if (nch > 0) chunk(gb, w0);
+            const double2* Sr = reinterpret_cast<const double2*>(Sb + buf * ldS + k0);
+            double acc[4] = {0, 0, 0, 0};
+#pragma unroll
+            for (int q2 = 0; q2 < RR / 2; ++q2) {
+                const double2 v = Sr[q2];
+                acc[(2 * q2) & 3] = fma(wr[2 * q2 + 0], v.x, acc[(2 * q2) & 3]);
+                acc[(2 * q2 + 1) & 3] = fma(wr[2 * q2 + 1], v.y, acc[(2 * q2 + 1) & 3]);
+            }
+            for (int q2 = 0; q2 < LL / 2; ++q2) {
+                const double2 v = Sr[RR / 2 + q2];
+                const double2 w = SCl[(size_t)q2 * T + tid];
+                acc[(2 * q2) & 3] = fma(w.x, v.x, acc[(2 * q2) & 3]);
+                acc[(2 * q2 + 1) & 3] = fma(w.y, v.y, acc[(2 * q2 + 1) & 3]);
+            }
+            const double2* Sg = Sr + (RR + LL) / 2;
+            for (int c = 0; c < nch; c += 2) {
+                if (c + 1 < nch) chunk(gb + c + 1, w1);
+#pragma unroll
+                for (int q2 = 0; q2 < CH / 2; ++q2) {
+                    const double2 v = Sg[c * (CH / 2) + q2];
+                    acc[(2 * q2) & 3] = fma(w0[2 * q2 + 0], v.x, acc[(2 * q2) & 3]);
+                    acc[(2 * q2 + 1) & 3] = fma(w0[2 * q2 + 1], v.y, acc[(2 * q2 + 1) & 3]);
+                }
+                if (c + 2 < nch) chunk(gb + c + 2, w0);
+                if (c + 1 < nch) {
+#pragma unroll
+                    for (int q2 = 0; q2 < CH / 2; ++q2) {
+                        const double2 v = Sg[(c + 1) * (CH / 2) + q2];
+                        acc[(2 * q2) & 3] = fma(w1[2 * q2 + 0], v.x, acc[(2 * q2) & 3]);
+                        acc[(2 * q2 + 1) & 3] = fma(w1[2 * q2 + 1], v.y, acc[(2 * q2 + 1) & 3]);
+                    }
+                }
+            }
+            double c = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+            if (KS > 1) {
+                if (ks > 0) part[ks * NW * 64 + j] = c;
+                __syncthreads();
+                if (ks == 0)
+                    for (int t = 1; t < KS; ++t) c += part[t * NW * 64 + j];
+            }
+            if (comp) {
+                Node<double>::step(c, gp, L.wEE, L.wEI, L.wIE, (double)nz.x, (double)nz.y, L.SE, L.SI, L.IE,
+                                   L.II, L.rE, L.rI);
+                Sb[(buf ^ 1) * ldS + j] = L.SE;
+            }
+            __syncthreads();
+            buf ^= 1;
+        }
+        if (comp) L.bw.step(L.SE);
+        if (ts % a.bold_tr == 0) {
+            if (comp) sample(a, sim, j, vol, L);
+            ++vol;
+        }
+        ++ts;
+        if (adjusting) {
+            if (comp && ts >= a.i_start && ts <= a.i_end) L.mIE += L.IE;
+            if (ts == a.i_end) {
+                const bool out = comp && fic_decide<double>(L.mIE, dur, trial, a.max_fic_trials, L.wIE, L.delta);
+                if (__syncthreads_or(out)) {
+                    if (trial < a.max_fic_trials) {
+                        if (comp) { L.reset(); Sb[buf * ldS + j] = 0.001; }
+                        ts = 0; vol = 0; ++trial;
+                        __syncthreads();
+                    } else {
+                        failed = true; adjusting = false;
+                    }
+                } else {
+                    adjusting = false;
+                }
+            }
+        }
+    }
+    finish(a, sim, j, comp, vol, trial, failed, L);
+}
+
+// ---------------------------------------------------------------------------
 // split_g: the split kernel for node counts whose SC does not fit in
 // registers (N > 256).  Lane j streams its SC column slice from a transposed
 // copy SCt[k][j] in global memory (L2-resident; consecutive lanes read
@@ -1385,6 +1532,50 @@ hipError_t launch_sim_hyb(const float* SCc, const SimArgs& a, int KS, size_t lds
         case 128: launch_hyb<128>(SCc, a, s, form, st); break;
         case 144: launch_hyb<144>(SCc, a, s, form, st); break;
         case 160: launch_hyb<160>(SCc, a, s, form, st); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// FP64 hybrid shape: as hyb_shape with 8-byte values in 8-value chunks; the
+// register part fits without spills up to 48 values at two waves per SIMD
+// and 64 at one
+constexpr int hyb64_maxt(int RR) { return RR <= 48 ? 512 : 256; }
+
+FastShape hyb64_shape(int N, int KS, size_t lds_bytes, int rr) {
+    FastShape s = stream_shape(N, KS, sizeof(double));
+    s.KC = rr >= 0 ? rr : s.threads <= 256 ? 64 : s.threads <= 512 ? 48 : 0;
+    while (s.KC > s.kc) s.KC -= 8;
+    const size_t base = (2 * (size_t)KS * s.kc + (size_t)KS * s.NW * 64) * sizeof(double);
+    int LL = 0;
+    if (lds_bytes > base + 16) LL = (int)((lds_bytes - base) / ((size_t)s.threads * sizeof(double)));
+    LL = std::min(LL, s.kc - s.KC) / 8 * 8;
+    s.SS = LL;
+    s.smem = base + (size_t)LL * s.threads * sizeof(double);
+    s.ok = s.threads <= 1024 && s.KC > 0 && s.smem <= lds_bytes && (s.kc - s.KC - LL) % 8 == 0 &&
+           s.KC % 8 == 0 && s.KC <= 64 && s.threads <= hyb64_maxt(s.KC);
+    return s;
+}
+
+template <int RR>
+static void launch_hyb64(const double* SCc, const SimArgs& a, const FastShape& s, hipStream_t st) {
+    constexpr int MAXT = hyb64_maxt(RR);
+    hipLaunchKernelGGL((k_sim_hyb64<RR, MAXT>), dim3(a.B), dim3(s.threads), s.smem, st, SCc, a, s.NW, s.KS, s.kc,
+                       s.SS);
+}
+
+hipError_t launch_sim_hyb64(const double* SCc, const SimArgs& a, int KS, size_t lds_bytes, int rr, hipStream_t st) {
+    const FastShape s = hyb64_shape(a.N, KS, lds_bytes, rr);
+    if (!s.ok) return hipErrorInvalidValue;
+    switch (s.KC) {
+        case 8: launch_hyb64<8>(SCc, a, s, st); break;
+        case 16: launch_hyb64<16>(SCc, a, s, st); break;
+        case 24: launch_hyb64<24>(SCc, a, s, st); break;
+        case 32: launch_hyb64<32>(SCc, a, s, st); break;
+        case 40: launch_hyb64<40>(SCc, a, s, st); break;
+        case 48: launch_hyb64<48>(SCc, a, s, st); break;
+        case 56: launch_hyb64<56>(SCc, a, s, st); break;
+        case 64: launch_hyb64<64>(SCc, a, s, st); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -380,12 +380,60 @@ def test_f64_streamed_sc_kernel_matches_oracle(case, ref, monkeypatch, stream_ks
     FP64 oracle."""
     monkeypatch.setenv("BNM_FORCE_STREAM", "1")
     monkeypatch.setenv("BNM_STREAM_KS", stream_ks)
+    monkeypatch.setenv("BNM_NO_HYB", "1")
     g = _gpu(case, _cfg(case, bnm.PREC_F64, bnm.KERNEL_FAST))
     np.testing.assert_array_equal(g["fic_trials"], ref["fic_trials"])
     rel = _bold_rel(g["bold"], ref["bold"])
     print("stream KS", stream_ks, "per-TR BOLD rel", rel)
     assert rel <= 1e-6
     np.testing.assert_allclose(g["corr"], ref["corr"], atol=1e-9)
+
+
+@pytest.mark.parametrize("N_big,ks", [(200, 1), (200, 2), (400, 1)])
+def test_f64_hybrid_vs_streamed(monkeypatch, N_big, ks):
+    """FP64 kernel with register- and LDS-resident SC and the rest streamed
+    (8-value chunks) against the FP64 streaming kernel: the same FIC trials
+    and FP64-level agreement (the two sum the coupling dot in different
+    orders).  Values per lane in VGPRs / LDS / streamed: N = 200 KS = 1
+    64 / 72 / 72, KS = 2 48 / 32 / 32; N = 400 KS = 1 48 / 40 / 312."""
+    print("case: f64 hybrid N", N_big, "KS", ks, flush=True)
+    B4, T4 = 4, 30_000
+    SC = W.synthetic_sc(N_big, 0.3, 19)
+    rng = np.random.default_rng(6)
+    G = np.array([0.8, 1.5, 2.5, 0.5])
+    wEE = W.heterogeneous(np.array([0.2, 0.25, 0.15, 0.3]), rng.normal(0, 1, N_big), 0.02)
+    wEI = W.heterogeneous(np.array([0.15, 0.2, 0.25, 0.1]), rng.normal(0, 1, N_big), 0.02)
+    noise = bnm.make_noise_chunked(8, (T4 + 1) * 20 * N_big)
+    emp = (rng.uniform(-0.2, 0.8, bnm.n_pairs(N_big, False)), rng.uniform(-0.5, 1, 40))
+    out = {}
+    for hyb in (True, False):
+        monkeypatch.delenv("BNM_HYB_KS", raising=False)
+        monkeypatch.delenv("BNM_NO_HYB", raising=False)
+        if hyb:
+            monkeypatch.setenv("BNM_HYB_KS", str(ks))
+        else:
+            monkeypatch.setenv("BNM_NO_HYB", "1")
+            monkeypatch.setenv("BNM_STREAM_KS", str(ks))
+            monkeypatch.setenv("BNM_FORCE_STREAM", "1")
+        cfg = bnm.default_config(nodes=N_big, n_sims=B4, time_steps=T4, bold_tr=1000, n_vols_remove=0,
+                                 window_size=10, window_step=2, exc_interhemispheric=0,
+                                 noise_period_ms=T4 + 1, precision=bnm.PREC_F64, kernel=bnm.KERNEL_FAST,
+                                 fc_precision=bnm.FC_F64)
+        with bnm.Context(cfg) as c:
+            c.set_sc(SC); c.set_noise(noise); c.set_empirical(*emp)
+            c.set_params(G, wEE, wEI)
+            c.run()
+            name = c.kernel_name()
+            print(name, c.timings())
+            assert name.startswith("hyb_f64(KS=%d" % ks if hyb else "stream_f64(KS=%d" % ks)
+            out[hyb] = c.outputs()
+    a, b = out[True], out[False]
+    np.testing.assert_array_equal(a["fic_trials"], b["fic_trials"])
+    rel = _bold_rel(a["bold"][:, 1:], b["bold"][:, 1:])
+    print(f"N={N_big} KS={ks}: BOLD rel {rel:.3g}, FC {np.abs(a['fc'] - b['fc']).max():.3g}")
+    assert rel <= 1e-6
+    np.testing.assert_allclose(a["fc"], b["fc"], atol=1e-7)
+    np.testing.assert_allclose(a["corr"], b["corr"], atol=1e-7)
 
 
 def test_n400_streamed_f32_vs_f64():
@@ -411,7 +459,7 @@ def test_n400_streamed_f32_vs_f64():
             c.set_params(G, wEE, wEI)
             c.run()
             print(c.kernel_name(), c.timings())
-            assert c.kernel_name().startswith("stream" if prec == bnm.PREC_F64 else ("stream", "hyb"))
+            assert c.kernel_name().startswith(("stream", "hyb"))
             out[prec] = c.outputs()
     a, b = out[bnm.PREC_F32], out[bnm.PREC_F64]
     np.testing.assert_array_equal(a["fic_trials"], b["fic_trials"])
