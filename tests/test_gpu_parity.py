@@ -397,7 +397,7 @@ def test_f64_hybrid_vs_streamed(monkeypatch, N_big, ks):
     orders).  Values per lane in VGPRs / LDS / streamed: N = 200 KS = 1
     64 / 72 / 72, KS = 2 48 / 32 / 32; N = 400 KS = 1 48 / 40 / 312."""
     print("case: f64 hybrid N", N_big, "KS", ks, flush=True)
-    B4, T4 = 4, 30_000
+    B4, T4 = 4, 16_000
     SC = W.synthetic_sc(N_big, 0.3, 19)
     rng = np.random.default_rng(6)
     G = np.array([0.8, 1.5, 2.5, 0.5])
