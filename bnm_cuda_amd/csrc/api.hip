@@ -723,9 +723,11 @@ static std::vector<Variant> candidates(const bnm_ctx* x) {
         }
         if (!x->f64 && !x->no_ei) {
             if (x->ei_bc == 1) add({4, 4, 1, 1});
-            // forms 3 and 2 (short chain) are tuned by default; form 0 only when pinned
+            // form 3 (short chain, one partial sum) is tuned by default; forms
+            // 2 and 0 (within ~1.5 % / 4 %, below the tuner's window noise)
+            // only when pinned
             for (int f : {3, 2, 0})
-                if (x->ei_form < 0 ? f != 0 : x->ei_form == f) {
+                if (x->ei_form < 0 ? f == 3 : x->ei_form == f) {
                     add({4, 4, 1, f});
                     add({4, 2, 1, f});
                 }
