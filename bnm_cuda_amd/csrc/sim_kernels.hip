@@ -1419,15 +1419,19 @@ hipError_t launch_sim_ei(const float* SC, const SimArgs& a, int LS, int form, hi
     return hipGetLastError();
 }
 
+// FP64 E/I workgroup bound: the kernel keeps its 28 SC doubles per lane and
+// state within 256 VGPRs (two waves per SIMD) without spills up to KC = 28
+constexpr int ei64_maxt(int KC) { return KC <= 28 ? 512 : 256; }
+
 template <int KC, int LS>
 static void launch_ei64(const double* SC, const SimArgs& a, const FastShape& s, hipStream_t st) {
-    constexpr int MAXT = maxt_for(KC, 8);
+    constexpr int MAXT = ei64_maxt(KC);
     hipLaunchKernelGGL((k_sim_ei64<KC, LS, MAXT>), dim3(a.B), dim3(s.threads), s.smem, st, SC, a, s.NW, s.SS);
 }
 
 FastShape ei64_shape(int N, int LS) {
     FastShape s = fast_shape(N, LS, 1, sizeof(double));
-    s.ok = s.ok && (LS == 2 || LS == 4);
+    s.ok = s.KC > 0 && s.threads <= ei64_maxt(s.KC) && s.smem <= 64 * 1024 && (LS == 2 || LS == 4);
     return s;
 }
 
