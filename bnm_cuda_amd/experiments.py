@@ -183,8 +183,11 @@ def fcd_stress(B=16, N=400, T=864_000, TR=720, ws=44, step=1):
         kname = c.kernel_name()
     _log(f"[fcd400] integration {time.perf_counter() - t0:.1f} s ({kname}), phases {t_int}")
     r_ref = _pearson_rows(ref["fc"], emp[0])
-    flop_fc = 2.0 * N * N * (ref["bold"].shape[1] + (ws // 2 * 2 + 1) * nw) * B
-    flop_fcd = 2.0 * nw * nw * np_ * B
+    # algorithmic work: one multiply-add per pair and sample over the lower
+    # triangle that FC / FCD need (N(N-1)/2 node pairs over the whole run and
+    # every window; nw(nw-1)/2 window pairs over np_ edges)
+    flop_fc = 1.0 * N * (N - 1) * (ref["bold"].shape[1] + (ws // 2 * 2 + 1) * nw) * B
+    flop_fcd = 1.0 * nw * (nw - 1) * np_ * B
     rows = {}
     variants = [("f64", _lib.FC_F64, False), ("f32", _lib.FC_F32, False), ("bf16", _lib.FC_BF16, False),
                 ("bf16x3", _lib.FC_BF16X3, False), ("f32_direct", _lib.FC_F32, True),
