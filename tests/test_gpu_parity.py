@@ -476,7 +476,7 @@ def test_hybrid_resident_sc_kernel_vs_f64(monkeypatch, N_big, ks, form):
     early BOLD close (a wrong chunk offset or LDS slot would not be).
     form 2: three streamed chunks in flight, LDS chunks interleaved."""
     monkeypatch.setenv("BNM_HYB_BC", str(form))
-    B4, T4 = 4, 30_000
+    B4, T4 = 4, 16_000
     SC = W.synthetic_sc(N_big, 0.3, 17)
     rng = np.random.default_rng(5)
     G = np.array([0.8, 1.5, 2.5, 0.5])
