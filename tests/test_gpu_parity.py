@@ -418,7 +418,7 @@ def test_f64_hybrid_vs_streamed(monkeypatch, N_big, ks):
         cfg = bnm.default_config(nodes=N_big, n_sims=B4, time_steps=T4, bold_tr=1000, n_vols_remove=0,
                                  window_size=10, window_step=2, exc_interhemispheric=0,
                                  noise_period_ms=T4 + 1, precision=bnm.PREC_F64, kernel=bnm.KERNEL_FAST,
-                                 fc_precision=bnm.FC_F64)
+                                 max_fic_trials=2, fc_precision=bnm.FC_F64)
         with bnm.Context(cfg) as c:
             c.set_sc(SC); c.set_noise(noise); c.set_empirical(*emp)
             c.set_params(G, wEE, wEI)
@@ -453,7 +453,7 @@ def test_n400_streamed_f32_vs_f64():
         cfg = bnm.default_config(nodes=N4, n_sims=B4, time_steps=T4, bold_tr=720, n_vols_remove=0,
                                  window_size=10, window_step=2, exc_interhemispheric=0,
                                  noise_period_ms=T4 + 1, precision=prec, kernel=bnm.KERNEL_FAST,
-                                 fc_precision=bnm.FC_F64 if prec == bnm.PREC_F64 else bnm.FC_F32)
+                                 max_fic_trials=2, fc_precision=bnm.FC_F64 if prec == bnm.PREC_F64 else bnm.FC_F32)
         with bnm.Context(cfg) as c:
             c.set_sc(SC); c.set_noise(noise); c.set_empirical(emp_fc, emp_fcd)
             c.set_params(G, wEE, wEI)
@@ -492,7 +492,7 @@ def test_hybrid_resident_sc_kernel_vs_f64(monkeypatch, N_big, ks, form):
         cfg = bnm.default_config(nodes=N_big, n_sims=B4, time_steps=T4, bold_tr=1000, n_vols_remove=0,
                                  window_size=10, window_step=2, exc_interhemispheric=0,
                                  noise_period_ms=T4 + 1, precision=prec, kernel=bnm.KERNEL_FAST,
-                                 fc_precision=bnm.FC_F64)
+                                 max_fic_trials=2, fc_precision=bnm.FC_F64)
         with bnm.Context(cfg) as c:
             c.set_sc(SC); c.set_noise(noise); c.set_empirical(emp_fc, emp_fcd)
             c.set_params(G, wEE, wEI)
