@@ -79,6 +79,7 @@ struct bnm_ctx {
     bool fcbf16 = false;  // BF16 Gram operands (FP32 accumulation and storage)
     bool fcsplit = false; // ... as hi/lo pairs (three products)
     bool fcd_direct = false;
+    int fcd_tile = 64;
     bool have_bold = false;  // BOLD holds a run's or a caller's series
     hipStream_t st = nullptr;
     double* SC64 = nullptr;
@@ -453,6 +454,9 @@ bnm_ctx* bnm_ctx_create(const bnm_config* cfg, int* status) {
     // BNM_FCD_DIRECT=1: z-score inside the FCD Gram's tile loader instead of
     // a separate pass (no z buffer; slower, kept for comparison)
     x->fcd_direct = env_int("BNM_FCD_DIRECT", 0) != 0;
+    // BNM_FCD_TILE=128: the FP32 FCD Gram over 128x128 output tiles (bit-identical;
+    // slower at fcd400: 28.8 vs 24.8 ms, a third as many workgroups)
+    x->fcd_tile = env_int("BNM_FCD_TILE", 64);
     if (!c.sim_only) {
         x->np = bnm_n_pairs(c.nodes, c.exc_interhemispheric);
         x->nw = c.no_fcd ? 0 : bnm_n_windows(&c);
@@ -911,7 +915,7 @@ static int enqueue_run(bnm_ctx* x, hipStream_t st, bool timed, bool analyze_only
                                              x->wmean, x->winv, st));
             if (x->wz)
                 HIPCHK(launch_fcd_z((const float*)x->wfc, x->B, x->nw, x->np, x->wmean, x->winv, x->wz, x->wzlo,
-                                    x->fcsplit ? 2 : x->fcbf16 ? 1 : 0, (float*)x->fcd, st));
+                                    x->fcsplit ? 2 : x->fcbf16 ? 1 : 0, (float*)x->fcd, st, x->fcd_tile));
             else if (x->fcbf16)
                 HIPCHK(launch_gram_fcd_bf16((const float*)x->wfc, x->B, x->nw, x->np, x->wmean, x->winv,
                                             (float*)x->fcd, x->fcsplit, st));

@@ -548,8 +548,78 @@ __global__ __launch_bounds__(256) void k_gram_zbf16(const unsigned short* __rest
             }
 }
 
+// FP32 operands, 128x128 output tiles: each wave computes a 64x64 block as a
+// 4x4 grid of 16x16 MFMA tiles, so a K step reads 8 fragments from LDS for
+// 16 MFMAs (the 64x64 tile reads 4 for 4), and every z-scored row is read
+// from memory by half as many tiles.  In a diagonal tile the wave whose block
+// lies wholly above the diagonal (rows 0-63, columns 64-127) only loads.
+__global__ __launch_bounds__(256) void k_gram_z32_t128(const float* __restrict__ Z, int nw, int npp,
+                                                       float* __restrict__ fcd) {
+    constexpr int TT = 128;
+    __shared__ __align__(16) float As[TT][ZLD32];
+    __shared__ __align__(16) float Bs[TT][ZLD32];
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    int ti, tj;
+    tile_pair(blockIdx.x, ti, tj);
+    const int sim = blockIdx.z;
+    const float* Zs = Z + (size_t)sim * nw * npp;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wi = w >> 1, wj = w & 1;
+    const bool idle = ti == tj && wi < wj;   // block above the diagonal
+    v4 acc[4][4];
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b) acc[a][b] = v4{0, 0, 0, 0};
+    // loader: 128 rows x 32 k = 1024 float4 per operand, four per thread
+    const int lr = tid >> 3, lc = (tid & 7) * 4;   // rows lr + 32q, k lc..lc+3
+    auto ld = [&](int t, int r, int k0) -> float4 {
+        const int row = t * TT + r;
+        return row < nw ? *reinterpret_cast<const float4*>(Zs + (size_t)row * npp + k0 + lc)
+                        : float4{0, 0, 0, 0};
+    };
+    float4 ra[4], rb[4];
+    for (int q = 0; q < 4; ++q) { ra[q] = ld(ti, lr + 32 * q, 0); rb[q] = ld(tj, lr + 32 * q, 0); }
+    for (int k0 = 0; k0 < npp; k0 += ZK32) {
+        for (int q = 0; q < 4; ++q) {
+            *reinterpret_cast<float4*>(&As[lr + 32 * q][lc]) = ra[q];
+            *reinterpret_cast<float4*>(&Bs[lr + 32 * q][lc]) = rb[q];
+        }
+        __syncthreads();
+        if (k0 + ZK32 < npp)   // next chunk in flight behind the MFMAs
+            for (int q = 0; q < 4; ++q) {
+                ra[q] = ld(ti, lr + 32 * q, k0 + ZK32);
+                rb[q] = ld(tj, lr + 32 * q, k0 + ZK32);
+            }
+        if (!idle) {
+#pragma unroll
+            for (int s = 0; s < ZK32 / 4; ++s) {
+                float a[4], b[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    a[q] = As[wi * 64 + q * 16 + (lane & 15)][4 * s + (lane >> 4)];
+                    b[q] = Bs[wj * 64 + q * 16 + (lane & 15)][4 * s + (lane >> 4)];
+                }
+#pragma unroll
+                for (int qa = 0; qa < 4; ++qa)
+#pragma unroll
+                    for (int qb = 0; qb < 4; ++qb)
+                        acc[qa][qb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[qa], b[qb], acc[qa][qb], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    if (idle) return;
+    float* out = fcd + (size_t)sim * ((long long)nw * (nw - 1) / 2);
+    for (int qa = 0; qa < 4; ++qa)
+        for (int qb = 0; qb < 4; ++qb)
+            for (int r = 0; r < 4; ++r) {
+                const int row = ti * TT + wi * 64 + qa * 16 + 4 * (lane >> 4) + r;
+                const int col = tj * TT + wj * 64 + qb * 16 + (lane & 15);
+                if (row < nw && col < row) out[(long long)row * (row - 1) / 2 + col] = acc[qa][qb][r];
+            }
+}
+
 hipError_t launch_fcd_z(const float* wfc, int B, int nw, int np, const double* mean, const double* inv_ssd,
-                        void* zhi, unsigned short* zlo, int mode, float* fcd, hipStream_t st) {
+                        void* zhi, unsigned short* zlo, int mode, float* fcd, hipStream_t st, int tile) {
     const int npp = (np + 63) / 64 * 64;
     const unsigned rows = (unsigned)((long long)B * nw);
     if (mode == 0)
@@ -560,7 +630,11 @@ hipError_t launch_fcd_z(const float* wfc, int B, int nw, int np, const double* m
         hipLaunchKernelGGL(k_zscore_rows<2>, dim3(rows), dim3(256), 0, st, wfc, np, npp, mean, inv_ssd, zhi, zlo);
     const int nt = (nw + ZT - 1) / ZT;
     const dim3 grid(nt * (nt + 1) / 2, 1, B);
-    if (mode == 0)
+    const int nt2 = (nw + 127) / 128;
+    if (mode == 0 && tile == 128)
+        hipLaunchKernelGGL(k_gram_z32_t128, dim3(nt2 * (nt2 + 1) / 2, 1, B), dim3(256), 0, st, (const float*)zhi,
+                           nw, npp, fcd);
+    else if (mode == 0)
         hipLaunchKernelGGL(k_gram_z32, grid, dim3(256), 0, st, (const float*)zhi, nw, npp, fcd);
     else if (mode == 1)
         hipLaunchKernelGGL(k_gram_zbf16<false>, grid, dim3(256), 0, st, (const unsigned short*)zhi, zlo, nw, npp,

@@ -106,8 +106,9 @@ hipError_t launch_gram_fcd_bf16(const float* wfc, int B, int nw, int np, const d
 // FCD from z-scored rows: z-scores the FP32 window-FC rows into zhi (FP32
 // [B*nw][npp], or BF16 hi) and zlo (BF16 lo, mode 2), npp = np rounded up to
 // 64, then Grams them (mode 0 FP32, 1 BF16, 2 BF16 hi/lo three products)
+// (tile 128: FP32 Gram over 128x128 output tiles; otherwise 64x64, the default)
 hipError_t launch_fcd_z(const float* wfc, int B, int nw, int np, const double* mean, const double* inv_ssd,
-                        void* zhi, unsigned short* zlo, int mode, float* fcd, hipStream_t st);
+                        void* zhi, unsigned short* zlo, int mode, float* fcd, hipStream_t st, int tile = 64);
 // Row moments of the window-FC matrix (mean, 1/ssd over the np edges).
 template <class T>
 hipError_t launch_row_moments(const T* x, long long rows, int len, double* mean, double* inv_ssd,

@@ -191,10 +191,12 @@ def fcd_stress(B=16, N=400, T=864_000, TR=720, ws=44, step=1):
     rows = {}
     variants = [("f64", _lib.FC_F64, False), ("f32", _lib.FC_F32, False), ("bf16", _lib.FC_BF16, False),
                 ("bf16x3", _lib.FC_BF16X3, False), ("f32_direct", _lib.FC_F32, True),
-                ("bf16x3_direct", _lib.FC_BF16X3, True)]
+                ("bf16x3_direct", _lib.FC_BF16X3, True), ("f32_tile128", _lib.FC_F32, False)]
     for name, fcp, direct in variants:
-        # direct: the FCD Gram z-scores inside its tile loader (read at context creation)
+        # direct: the FCD Gram z-scores inside its tile loader; tile128: the FP32
+        # Gram over 128x128 output tiles (both read at context creation)
         os.environ["BNM_FCD_DIRECT"] = "1" if direct else "0"
+        os.environ["BNM_FCD_TILE"] = "128" if name == "f32_tile128" else "64"
         with _lib.Context(_lib.default_config(**base, fc_precision=fcp)) as c:
             c.set_empirical(*emp)
             c.set_bold(ref["bold"])
@@ -203,6 +205,7 @@ def fcd_stress(B=16, N=400, T=864_000, TR=720, ws=44, step=1):
             t = c.timings()
             o = c.outputs(bold=False)
         os.environ.pop("BNM_FCD_DIRECT")
+        os.environ.pop("BNM_FCD_TILE")
         ks_err = np.abs((o["corr"] - (1 - _pearson_rows(o["fc"], emp[0])) - np.abs(emp[0].mean() - o["fc"].mean(1)))
                         - (ref["corr"] - (1 - r_ref) - np.abs(emp[0].mean() - ref["fc"].mean(1))))
         rows[name] = {"fc_ms": round(t["fc"], 3), "fcd_ms": round(t["fcd"], 3), "fit_ms": round(t["fit"], 3),

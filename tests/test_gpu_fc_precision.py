@@ -87,3 +87,27 @@ def test_fcd_zscored_pass_matches_loader_zscoring(monkeypatch, fcp):
     print(f"fcd max |z-pass - loader| {d:.2e}")
     assert d <= 2e-5
     assert np.abs(oz["corr"] - od["corr"]).max() <= 1e-3
+
+
+def test_fcd_gram_128_tiles_match_64_tiles(monkeypatch):
+    """The FP32 FCD Gram over 128x128 output tiles against 64x64 tiles (the
+    default): each output element accumulates the same products in the same
+    order, so the FCD is bit-identical.  ~290 windows span three 128-row
+    tiles (diagonal, off-diagonal, ragged) and five 64-row tiles."""
+    N, T, TR, ws, step, B = 20, 300_000, 1000, 10, 1, 3
+    SC = W.synthetic_sc(N, 0.3, 13)
+    g, ee, ei = W.population(B, 14)
+    params = (g, W.expand_homogeneous(ee, N), W.expand_homogeneous(ei, N))
+    noise = bnm.make_noise_chunked(15, (T + 1) * 20 * N)
+    rng = np.random.default_rng(16)
+    emp = (rng.uniform(-0.2, 0.8, bnm.n_pairs(N, False)), rng.uniform(-0.5, 1.0, 3000))
+    out = {}
+    for tile in ("128", "64"):
+        print("case: fcd gram tile", tile, flush=True)
+        monkeypatch.setenv("BNM_FCD_TILE", tile)
+        out[tile] = _run(bnm.FC_F32, N, B, T, TR, ws, step, SC, params, noise, emp)
+    nw = (1 + int(np.sqrt(1 + 8 * out["128"]["fcd"].shape[1]))) // 2
+    print("windows", nw)
+    assert nw > 256
+    np.testing.assert_array_equal(out["128"]["fcd"], out["64"]["fcd"])
+    np.testing.assert_array_equal(out["128"]["corr"], out["64"]["corr"])
