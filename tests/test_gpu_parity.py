@@ -389,6 +389,53 @@ def test_f64_streamed_sc_kernel_matches_oracle(case, ref, monkeypatch, stream_ks
     np.testing.assert_allclose(g["corr"], ref["corr"], atol=1e-9)
 
 
+@pytest.mark.parametrize("N_r", [150, 257])
+def test_hybrid_ragged_node_counts(monkeypatch, N_r):
+    """Node counts that are not multiples of 16 or 64: every FP32 hybrid
+    form and the FP64 hybrid against the FP64 streaming kernel (the padded
+    chunk rows of the chunked SC, the padded LDS slots and the broadcast
+    reads of padded S_E slots all take part)."""
+    print("case: hybrid ragged N", N_r, flush=True)
+    B2, T2 = 2, 12_000
+    SC = W.synthetic_sc(N_r, 0.35, 41)
+    rng = np.random.default_rng(42)
+    G = np.array([1.2, 2.2])
+    wEE = W.heterogeneous(np.array([0.2, 0.25]), rng.normal(0, 1, N_r), 0.02)
+    wEI = W.heterogeneous(np.array([0.15, 0.2]), rng.normal(0, 1, N_r), 0.02)
+    noise = bnm.make_noise_chunked(43, (T2 + 1) * 20 * N_r)
+    emp = (rng.uniform(-0.2, 0.8, bnm.n_pairs(N_r, False)), rng.uniform(-0.5, 1, 40))
+
+    def run(prec, env):
+        for k in ("BNM_HYB_KS", "BNM_HYB_BC", "BNM_NO_HYB", "BNM_FORCE_STREAM", "BNM_STREAM_KS"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        cfg = bnm.default_config(nodes=N_r, n_sims=B2, time_steps=T2, bold_tr=1000, n_vols_remove=0,
+                                 window_size=6, window_step=2, exc_interhemispheric=0, noise_period_ms=T2 + 1,
+                                 precision=prec, kernel=bnm.KERNEL_FAST, max_fic_trials=2,
+                                 fc_precision=bnm.FC_F64)
+        with bnm.Context(cfg) as c:
+            c.set_sc(SC); c.set_noise(noise); c.set_empirical(*emp)
+            c.set_params(G, wEE, wEI)
+            c.run()
+            return c.kernel_name(), c.outputs()
+
+    kr, ref = run(bnm.PREC_F64, {"BNM_NO_HYB": "1", "BNM_FORCE_STREAM": "1", "BNM_STREAM_KS": "1"})
+    assert kr.startswith("stream_f64")
+    k64, h64 = run(bnm.PREC_F64, {"BNM_HYB_KS": "1"})
+    assert k64.startswith("hyb_f64")
+    np.testing.assert_array_equal(h64["bold"], ref["bold"])
+    for form in ("0", "1", "2"):
+        kf, o = run(bnm.PREC_F32, {"BNM_HYB_KS": "1", "BNM_HYB_BC": form})
+        print(kf)
+        assert kf.startswith("hyb_f32(KS=1")
+        np.testing.assert_array_equal(o["fic_trials"], ref["fic_trials"])
+        rel = _bold_rel(o["bold"][:, 1:], ref["bold"][:, 1:])
+        print(f"N={N_r} form {form}: BOLD rel {rel:.3g}")
+        assert rel <= 1e-4
+        assert np.abs(o["fc"] - ref["fc"]).max() <= 1e-3
+
+
 @pytest.mark.parametrize("N_big,ks", [(200, 1), (200, 2), (400, 1)])
 def test_f64_hybrid_vs_streamed(monkeypatch, N_big, ks):
     """FP64 kernel with register- and LDS-resident SC and the rest streamed
