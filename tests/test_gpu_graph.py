@@ -113,3 +113,35 @@ def test_uneven_populations_in_one_graph_launch(monkeypatch, branches):
     m.close()
     for c in ctxs:
         c.close()
+
+
+def test_mixed_kernels_in_one_graph_launch(monkeypatch):
+    """One graph with an FP64 population, a BF16x3-coupling population on the
+    MFMA kernel (16 simulations per workgroup) and an FP32 population, packed
+    into two branches: each matches its own eager run bitwise."""
+    monkeypatch.setenv("BNM_MULTI_BRANCHES", "2")
+    specs = [(24, 5, 50_000, 7, bnm.PREC_F64, {}), (40, 20, 60_000, 8, bnm.PREC_BF16X3, {"kernel": bnm.KERNEL_MFMA}),
+             (30, 9, 40_000, 9, bnm.PREC_F32, {})]
+    ctxs, refs, names = [], [], []
+    for N, B, T, seed, prec, kw in specs:
+        print("case: mixed graph N", N, "B", B, "precision", prec, flush=True)
+        pop = _population(N, B, T, seed, prec, autotune=0, **kw)
+        e = _ctx(*pop)
+        e.run()
+        refs.append(e.outputs())
+        names.append(e.kernel_name())
+        e.close()
+        ctxs.append(_ctx(*pop))
+    print(names)
+    assert names[1].startswith("mfma_bf16x3")
+    m = bnm.Multi(ctxs)
+    plan = m.plan()
+    print(plan)
+    assert plan.count("branch") == 2 and "2 wg" in plan      # 20 simulations / 16 per workgroup
+    for rep in range(2):
+        m.run()
+        for c, r in zip(ctxs, refs):
+            _same(c.outputs(), r)
+    m.close()
+    for c in ctxs:
+        c.close()
