@@ -213,7 +213,7 @@ def surrogate_experiment(args):
                   "mean_fic_restarts": float(np.mean([float(r["mean_fic_trials"]) for r in plain])),
                   "best_rerun_plain_cost": plain_rerun},
         "surrogate": {"best_cost": float(surr[-1]["best_cost"]),
-                      "best_plain_judged_cost": float(surr[-1]["best_plain_cost"]),
+                      "best_plain_rerun_cost": float(surr[-1]["best_plain_cost"]),
                       "full_sims_total": int(surr[-1]["full_sims"]),
                       "wall_s": round(w_surr, 1),
                       "mean_fic_restarts": float(np.mean([float(r["mean_fic_trials"]) for r in surr])),
@@ -262,7 +262,7 @@ def main():
     if args.gpus != 1 or world != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE "
               f"{world}): simulations are independent, so a population would shard over GPUs "
-              "with no per-step exchange, but this build measures the one-GPU share.",
+              "with no per-step exchange; this program measures one GPU's share.",
               file=sys.stderr)
         sys.exit(2)
     if args.workload == "surrogate":

@@ -771,12 +771,12 @@ int main(int argc, char* argv[]) {
             check(bnm_ctx_get_outputs(jctx, c1.data(), p1.data(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
                                       nullptr, nullptr, nullptr, &t1, nullptr, nullptr),
                   "outputs");
-            const double judged = c1[0] + (do_fic ? p1[0] : 0.0);
+            const double plain_cost = c1[0] + (do_fic ? p1[0] : 0.0);
             ++full_sims;
             sim_seconds += (time_steps + 1 + (do_fic ? (double)t1 * (cfg.i_sampling_end + 1) : 0.0)) / 1000.0;
-            best_plain = std::min(best_plain, judged);
+            best_plain = std::min(best_plain, plain_cost);
             both(log, "surrogate: plain re-run of the generation best (IndPar %d): cost %f (search saw %f)\n", ib,
-                 judged, raw[ib]);
+                 plain_cost, raw[ib]);
         } else {
             best_plain = best_so_far;
         }

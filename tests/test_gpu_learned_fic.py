@@ -56,7 +56,7 @@ def test_learned_start_cuts_fic_restarts():
     assert t_l[ok].mean() < 0.6 * t_a[ok].mean()
     assert f_l.sum() <= f_a.sum() + 2
     assert d_l < 0.6 * d_an
-    # a learned start is still judged by the numerical FIC: every converged set
+    # a learned start is still decided by the numerical FIC: every converged set
     # ends in the band, so its wIE differs from the analytical-start run's by
     # at most the band's width in wIE (|dI_E| <= 0.01 at S_I ~ 0.04)
     assert np.abs(w_l - w_a)[ok].max() < 0.5
