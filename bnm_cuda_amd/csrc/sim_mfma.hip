@@ -137,11 +137,19 @@ __global__ __launch_bounds__(512) void k_sim_mfma(const float* __restrict__ SC, 
         const bool active = sim_ok && ts <= T;
         const int ms = ts % P;
         const float2* nzp = noise2 + (size_t)ms * 10 * N + j0;
-        for (int s = 0; s < 10; ++s) {
-            // noise of this substep for the lane's four nodes
-            float2 nz[4];
+        // noise of the lane's four nodes, one substep ahead: the loads for
+        // substep s + 1 are in flight behind substep s
+        float2 nz[4], nzn[4];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) nz[r] = node_ok[r] && active ? nzp[(size_t)s * N + r] : float2{0, 0};
+        for (int r = 0; r < 4; ++r) nzn[r] = node_ok[r] && active ? nzp[r] : float2{0, 0};
+        for (int s = 0; s < 10; ++s) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) nz[r] = nzn[r];
+            if (s < 9) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    nzn[r] = node_ok[r] && active ? nzp[(size_t)(s + 1) * N + r] : float2{0, 0};
+            }
             // ---- coupling on MFMA ------------------------------------------
             f32x4 acc = {0, 0, 0, 0};
             if constexpr (MODE == 0) {
