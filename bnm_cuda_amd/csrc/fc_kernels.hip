@@ -202,11 +202,9 @@ typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 constexpr int TKB = 32;          // K chunk
 constexpr int LDB = TKB + 8;     // u16 per staged row (80 B: rows stay 16-byte aligned)
 
-__device__ __forceinline__ unsigned short bf16_bits(float x) {
-    unsigned u = __float_as_uint(x);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (unsigned short)(u >> 16);
-}
+// gfx950's v_cvt_pk_bf16_f32: round to nearest even, the same bits as
+// adding 0x7fff + lsb to the FP32 pattern for finite inputs
+__device__ __forceinline__ unsigned short bf16_bits(float x) { return __builtin_bit_cast(unsigned short, (__bf16)x); }
 
 // SPLIT: each z-score is carried as hi = bf16(z) and lo = bf16(z - hi), and
 // the product as hi*hi + hi*lo + lo*hi (three MFMAs; the dropped lo*lo term
