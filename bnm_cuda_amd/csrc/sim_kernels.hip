@@ -704,9 +704,6 @@ __global__ __launch_bounds__(MAXT) void k_sim_ei64(const double* __restrict__ SC
 #pragma unroll
         for (int s = 0; s < 10; ++s) {
             const double2* Sr = reinterpret_cast<const double2*>(Sb + buf * ldS + sl * SS);
-            double2 sv[KC / 2];
-#pragma unroll
-            for (int q = 0; q < KC / 2; ++q) sv[q] = Sr[q];
             // previous-substep S of both pools: the lower row holds S_E, the
             // upper S_I (slices 2 and 3 mirror 0 and 1)
             double SEv, SIv;
@@ -723,8 +720,11 @@ __global__ __launch_bounds__(MAXT) void k_sim_ei64(const double* __restrict__ SC
             double acc[4] = {0, 0, 0, 0};
 #pragma unroll
             for (int q = 0; q < KC / 2; ++q) {
-                acc[(2 * q) & 3] = fma(sc[2 * q + 0], sv[q].x, acc[(2 * q) & 3]);
-                acc[(2 * q + 1) & 3] = fma(sc[2 * q + 1], sv[q].y, acc[(2 * q + 1) & 3]);
+                // reads issued by the scheduler as registers allow (all 26 of
+                // KC = 52 in flight at once would not fit beside the SC values)
+                const double2 v = Sr[q];
+                acc[(2 * q) & 3] = fma(sc[2 * q + 0], v.x, acc[(2 * q) & 3]);
+                acc[(2 * q + 1) & 3] = fma(sc[2 * q + 1], v.y, acc[(2 * q + 1) & 3]);
             }
             double c = (acc[0] + acc[1]) + (acc[2] + acc[3]);
             if constexpr (LS == 4) c = lane_sum16<double>(c);
