@@ -63,10 +63,14 @@ def test_autotuner_picks_a_timed_variant():
     assert np.abs(o["fc"] - o2["fc"]).max() < 1e-3
 
 
-@pytest.mark.parametrize("precision", [bnm.PREC_F64, bnm.PREC_F32])
-def test_graph_replay_matches_eager(precision):
-    print("case: graph replay N=20 B=6 T=60 s precision", precision, flush=True)
-    cfg, SC, params, noise, emp = _population(20, 6, 60_000, 2, precision, autotune=0)
+@pytest.mark.parametrize("precision,N,T", [(bnm.PREC_F64, 20, 60_000), (bnm.PREC_F32, 20, 60_000),
+                                           (bnm.PREC_F32, 200, 24_000), (bnm.PREC_F64, 200, 24_000)])
+def test_graph_replay_matches_eager(precision, N, T):
+    """Captured graph replays against eager runs, bitwise, with eager work in
+    between; N = 200 runs the hybrid kernels (over 64 KB of dynamic LDS)
+    inside the graph."""
+    print("case: graph replay N", N, "T", T, "precision", precision, flush=True)
+    cfg, SC, params, noise, emp = _population(N, 6 if N < 100 else 4, T, 2, precision, autotune=0)
     eager = _ctx(cfg, SC, params, noise, emp)
     eager.run()
     ref = eager.outputs()
