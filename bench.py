@@ -246,9 +246,11 @@ def main():
                     help="coupling precision of the timed path (profiling variants)")
     ap.add_argument("--kernel", default="auto", choices=["auto", "fast", "mfma"],
                     help="integration kernel family of the timed path (profiling variants)")
-    ap.add_argument("--workload", default="pop128", choices=["pop128", "surrogate", "uneven", "fcd400", "copy"],
+    ap.add_argument("--workload", default="pop128",
+                    choices=["pop128", "surrogate", "uneven", "fcd400", "scaling", "copy"],
                     help="pop128: the headline line; surrogate: CMA-ES with vs without the surrogate; "
-                         "uneven: four populations (N 68-400) in one graph launch; fcd400: FC/FCD stress")
+                         "uneven: four populations (N 68-400) in one graph launch; fcd400: FC/FCD stress; "
+                         "scaling: throughput against population size (N 100, 128-4096 sims)")
     ap.add_argument("--scale", type=float, default=1.0, help="duration scale (uneven workload)")
     ap.add_argument("--generations", type=int, default=32)
     ap.add_argument("--surrogate-lambda", type=int, default=64)
@@ -271,10 +273,14 @@ def main():
     if args.workload == "copy":
         copy_calibration()
         return
-    if args.workload in ("uneven", "fcd400"):
+    if args.workload in ("uneven", "fcd400", "scaling"):
         from bnm_cuda_amd import experiments as X
-        res = (X.uneven_batch(durations_scale=args.scale, fp64=not args.no_fp64, bf16x3=not args.no_fp64)
-               if args.workload == "uneven" else X.fcd_stress())
+        if args.workload == "uneven":
+            res = X.uneven_batch(durations_scale=args.scale, fp64=not args.no_fp64, bf16x3=not args.no_fp64)
+        elif args.workload == "scaling":
+            res = X.population_scaling()
+        else:
+            res = X.fcd_stress()
         print(json.dumps(res), flush=True)
         return
 

@@ -10,6 +10,9 @@ uneven_batch  four populations of different node counts and durations (N =
               one after another, with
               errors against FP64 runs and a BF16x3-coupling variant (MFMA kernel,
               N <= 128 populations).
+population_scaling  N = 100, 120 s runs of 128 to 4096 simulations (FP32
+              autotuned; BF16x3 coupling on the MFMA kernel), each run twice and
+              the second timed: throughput against population size.
 fcd_stress    N = 400, 16 simulations, 1201 volumes at TR 0.72 s, FCD window 44
               step 1 (1157 windows x 79,800 edges): the BOLD of one FP32
               integration is analysed with FP64, FP32, BF16 and BF16x3 Gram
@@ -157,6 +160,48 @@ def uneven_batch(B=128, durations_scale=1.0, fp64=True, bf16x3=True):
         res["error_vs_fp64"] = errs
         res["bounds"] = {"mean_rE_hz": 0.05, "fc": 1e-3, "fcd_ks": 0.01}
     return res
+
+
+def population_scaling(T=120_000, N=100,
+                       runs=((128, "f32", "auto"), (256, "f32", "auto"), (512, "f32", "auto"),
+                             (1024, "f32", "auto"), (2048, "f32", "auto"), (1024, "bf16x3", "mfma"),
+                             (2048, "bf16x3", "mfma"), (4096, "bf16x3", "mfma"), (4096, "f32", "auto"))):
+    """Throughput against population size at N = 100 (one SC, T = 120 s, FIC
+    on, FP32 autotuned or BF16x3 coupling on the MFMA kernel): nominal
+    simulated s per wall s of a whole run, second of two runs."""
+    SC = W.synthetic_sc(N, 0.3, 0)
+    _log(f"[scaling] noise {(T + 1) * 20 * N / 1e6:.0f} M draws")
+    noise = W.noise(N, T + 1, 1)
+    emp = (np.random.default_rng(0).uniform(-0.2, 0.8, _lib.n_pairs(N, True)),
+           np.random.default_rng(1).uniform(-0.5, 1, 300))
+    prec = {"f32": _lib.PREC_F32, "bf16x3": _lib.PREC_BF16X3}
+    kern = {"auto": _lib.KERNEL_AUTO, "mfma": _lib.KERNEL_MFMA}
+    rows = []
+    for B, p, k in runs:
+        g, ee, ei = W.population(B, 3)
+        cfg = _lib.default_config(nodes=N, n_sims=B, time_steps=T, bold_tr=1000, window_size=30, window_step=5,
+                                  noise_period_ms=T + 1, precision=prec[p], kernel=kern[k])
+        with _lib.Context(cfg) as c:
+            c.set_sc(SC)
+            c.set_noise(noise)
+            c.set_empirical(*emp)
+            c.set_params(g, W.expand_homogeneous(ee, N), W.expand_homogeneous(ei, N))
+            c.run()
+            c.run()
+            t = c.timings()
+            o = c.outputs(bold=False)
+            row = {"n_sims": B, "precision": p, "kernel": c.kernel_name(), "run_ms": round(float(t["total"]), 1),
+                   "sim_s_per_s": round(float(B * T / 1000 / (t["total"] / 1e3)), 1),
+                   "mean_fic_trials": float(o["fic_trials"].mean())}
+        _log(f"[scaling] {row}")
+        rows.append(row)
+    best = max(rows, key=lambda r: r["sim_s_per_s"])
+    return {"metric": "simulated s / wall s against population size (N=100, 120 s runs)",
+            "value": best["sim_s_per_s"], "unit": "simulated s / wall s", "higher_is_better": True,
+            "n_gpus": 1, "dtype": best["precision"], "data": "synthetic", "vs_baseline": None,
+            "config": {"workload": f"N={N}, one SC, T={T // 1000} s, FIC on; value = best row ({best['n_sims']} "
+                                   f"sims, {best['precision']})"},
+            "rows": rows}
 
 
 def fcd_stress(B=16, N=400, T=864_000, TR=720, ws=44, step=1):
