@@ -193,7 +193,8 @@ def surrogate_experiment(args):
     plain, w_plain, plain_rerun, _ = run("plain", lam, {})
     surr, w_surr, surr_rerun, surr_text = run("surrogate", args.surrogate_lambda,
                                               {"BNM_SURROGATE": "1", "BNM_SURR_POOL": str(args.pool),
-                                               "BNM_SURR_FIC": str(args.surrogate_fic)})
+                                               "BNM_SURR_FIC": str(args.surrogate_fic),
+                                               "BNM_SURR_UNSCREENED": str(args.unscreened)})
     # every cost compared here is a plain-method simulation: the plain run's
     # own costs, and the surrogate run's plain re-runs of its generation bests
     best_plain = float(plain[-1]["best_plain_cost"])
@@ -207,7 +208,7 @@ def surrogate_experiment(args):
         "config": {"workload": f"CMA-ES N=100, {T / 1000:.0f} s, 5 free parameters (G, wEE, wEI, "
                                f"weescale_0, weiscale_0), {args.generations} generations",
                    "plain_lambda": lam, "surrogate_lambda": args.surrogate_lambda, "pool": args.pool,
-                   "learned_fic_start": bool(args.surrogate_fic)},
+                   "learned_fic_start": bool(args.surrogate_fic), "unscreened": args.unscreened},
         "plain": {"best_cost": best_plain, "full_sims_total": sims_plain_total,
                   "full_sims_to_best": sims_plain_first, "wall_s": round(w_plain, 1),
                   "mean_fic_restarts": float(np.mean([float(r["mean_fic_trials"]) for r in plain])),
@@ -256,6 +257,8 @@ def main():
     ap.add_argument("--surrogate-lambda", type=int, default=64)
     ap.add_argument("--pool", type=int, default=8)
     ap.add_argument("--surrogate-fic", type=int, default=0, help="learned FIC start in the surrogate run")
+    ap.add_argument("--unscreened", type=float, default=0.25,
+                    help="fraction of each pre-screened generation drawn unscreened (surrogate workload)")
     ap.add_argument("--seed-mw", type=int, default=1)
     ap.add_argument("--window", type=int, default=30, help="FCD window (surrogate workload)")
     ap.add_argument("--step", type=int, default=5, help="FCD window step (surrogate workload)")

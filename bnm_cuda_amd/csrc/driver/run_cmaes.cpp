@@ -36,6 +36,11 @@
 //    instead (ridge least squares over the last BNM_SURR_LQ_POINTS simulated
 //    points, default max(2 x terms, 2 x lambda)); 2 ranks by the sum of the
 //    MLP's and the quadratic model's ranks.
+//  * BNM_SURR_ENSEMBLE=k trains k fit MLPs (different initialisation and
+//    shuffling seeds) on the same window and ranks by their mean predicted
+//    cost; BNM_SURR_UNSCREENED=f (default 0.25) takes round(f x lambda) of
+//    each pre-screened generation at random from the pool (an unscreened
+//    sample of the CMA-ES distribution) and the rest by predicted cost.
 //  * every reported cost is a full simulation; the best point is re-run with
 //    the plain method (analytical FIC init) at the end, as in the reference.
 #include <sys/stat.h>
@@ -555,7 +560,10 @@ int main(int argc, char* argv[]) {
     // and outputs standardised over that window (so the network resolves
     // the region the search has narrowed to), ranked by the cost output
     constexpr int KF = 5;
-    bnm_mlp* fit_net = nullptr;
+    bnm_mlp* fit_net = nullptr;          // first head of the ensemble
+    std::vector<bnm_mlp*> fit_nets;
+    const int n_heads = std::max(1, env_i("BNM_SURR_ENSEMBLE", 1));
+    const int n_unscreened = std::min(lambda, std::max(0, (int)std::lround(env_d("BNM_SURR_UNSCREENED", 0.25) * lambda)));
     std::vector<float> fit_X, fit_Y;
     std::vector<int> fit_gen;            // generation of each row
     Scaler fit_xs, fit_ys;
@@ -566,8 +574,12 @@ int main(int argc, char* argv[]) {
     const int lq_points = env_i("BNM_SURR_LQ_POINTS", 0);   // 0: max(2 * terms, 2 * lambda)
     LqModel lq;
     if (surrogate) {
-        fit_net = bnm_mlp_create(D, mlp_hidden, mlp_depth, KF, 1, 13, &st);
-        if (!fit_net) die(std::string("MLP: ") + bnm_mlp_last_error());
+        for (int h = 0; h < n_heads; ++h) {
+            bnm_mlp* m = bnm_mlp_create(D, mlp_hidden, mlp_depth, KF, 1, 13u + 1000u * h, &st);
+            if (!m) die(std::string("MLP: ") + bnm_mlp_last_error());
+            fit_nets.push_back(m);
+        }
+        fit_net = fit_nets[0];
     }
 
     bnm_ctx* sctx = nullptr;
@@ -630,9 +642,13 @@ int main(int argc, char* argv[]) {
             if (surr_model != 1) {
                 std::vector<float> px(pool.begin(), pool.end()), py((size_t)n_pool * KF);
                 fit_xs.apply(px, D);
-                check(bnm_mlp_predict(fit_net, px.data(), n_pool, py.data()), "surrogate predict");
+                std::fill(c.begin(), c.end(), 0.0);
+                for (bnm_mlp* m : fit_nets) {
+                    check(bnm_mlp_predict(m, px.data(), n_pool, py.data()), "surrogate predict");
+                    for (int i = 0; i < n_pool; ++i) c[i] += py[(size_t)i * KF + KF - 1];
+                }
                 for (int i = 0; i < n_pool; ++i)
-                    c[i] = fit_ys.mu[KF - 1] + fit_ys.sd[KF - 1] * py[(size_t)i * KF + KF - 1];
+                    c[i] = fit_ys.mu[KF - 1] + fit_ys.sd[KF - 1] * c[i] / n_heads;
             }
             if (surr_model != 0) {
                 for (int i = 0; i < n_pool; ++i) cl[i] = lq.predict(&pool[(size_t)i * D]);
@@ -644,7 +660,12 @@ int main(int argc, char* argv[]) {
             }
             std::vector<int> idx(n_pool);
             std::iota(idx.begin(), idx.end(), 0);
-            std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return c[a] < c[b]; });
+            // the unscreened part: a seeded random subset of the pool (the
+            // pool is an i.i.d. sample of the distribution), moved to the front
+            std::mt19937 pick(104729u * it + (unsigned)SeedMW);
+            for (int i = 0; i < n_unscreened; ++i)
+                std::swap(idx[i], idx[i + (int)(pick() % (unsigned)(n_pool - i))]);
+            std::stable_sort(idx.begin() + n_unscreened, idx.end(), [&](int a, int b) { return c[a] < c[b]; });
             idx.resize(lambda);
             check(bnm_cmaes_select(es, idx.data()), "select");
             for (int i = 0; i < lambda; ++i) pred_cost.push_back(c[idx[i]]);
@@ -723,11 +744,14 @@ int main(int argc, char* argv[]) {
             fit_xs.apply(wx, D);
             fit_ys.apply(wy, KF);
             if (surr_model != 1) {
-                check(bnm_mlp_set_data(fit_net, wx.data(), wy.data(), n), "surrogate data");
                 const int steps = env_i("BNM_SURR_FIT_STEPS", 1500);
                 const int batch = std::min(n, 128);
                 const int epochs = std::max(1, steps / std::max(1, n / batch));
-                check(bnm_mlp_train(fit_net, epochs, batch, 1e-3f, 17u + it, nullptr), "surrogate train");
+                for (int h = 0; h < n_heads; ++h) {
+                    check(bnm_mlp_set_data(fit_nets[h], wx.data(), wy.data(), n), "surrogate data");
+                    check(bnm_mlp_train(fit_nets[h], epochs, batch, 1e-3f, 17u + it + 7919u * h, nullptr),
+                          "surrogate train");
+                }
             }
             if (surr_model != 0) {
                 // the most recent points whose simulations produced a cost
@@ -859,7 +883,7 @@ int main(int argc, char* argv[]) {
     std::fclose(log);
     bnm_cmaes_destroy(es);
     if (fic_net) bnm_mlp_destroy(fic_net);
-    if (fit_net) bnm_mlp_destroy(fit_net);
+    for (bnm_mlp* m : fit_nets) bnm_mlp_destroy(m);
     if (sctx) bnm_ctx_destroy(sctx);
     if (jctx) bnm_ctx_destroy(jctx);
     bnm_ctx_destroy(ctx);

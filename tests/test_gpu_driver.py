@@ -141,3 +141,19 @@ def test_surrogate_mode_runs_and_reports(inputs, tmp_path):
     assert np.all(np.isfinite(rows["best_plain_cost"]))
     assert stdout.count("plain re-run of the generation best") == ITMAX - 1
     assert rows["surrogate_used"].tolist()[0] == 0 and rows["surrogate_used"].max() == 1
+
+
+def test_surrogate_ensemble_and_unscreened_fraction_run(inputs, tmp_path):
+    """Two fit heads averaged and 30 % of each pre-screened generation drawn
+    unscreened from the pool: the search runs through and every generation
+    is simulated in full."""
+    d, *_ = inputs
+    out = tmp_path / "surr_ens"
+    stdout = _run(d, out, {"BNM_SURROGATE": "1", "BNM_SURR_FIC": "0", "BNM_SURR_FIT_STEPS": "300",
+                           "BNM_SURR_ENSEMBLE": "2", "BNM_SURR_UNSCREENED": "0.3"})
+    assert len(_parse(stdout)) == (ITMAX - 1) * LAM
+    prog = [p for p in os.listdir(out / "cmaes_multimaps_gpu") if p.endswith("_progress.csv")][0]
+    rows = np.genfromtxt(out / "cmaes_multimaps_gpu" / prog, delimiter=",", names=True)
+    assert rows["full_sims"].tolist() == [LAM * (i + 1) for i in range(ITMAX - 1)]
+    assert np.all(np.isfinite(rows["best_plain_cost"]))
+    assert rows["surrogate_used"].tolist()[0] == 0 and rows["surrogate_used"].max() == 1
