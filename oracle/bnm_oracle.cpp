@@ -59,7 +59,9 @@ const double kI_E_ss = 0.3773805650, kS_E_ss = 0.1647572075;
 const double kRho = 0.34, kAlpha = 0.32, kTau = 0.98;
 const double kY = 1.0 / 0.41, kKappa = 1.0 / 0.65, kV0 = 0.02 * 100;
 const double kIAlpha = 1.0 / kAlpha, kITau = 1.0 / kTau, kOneMinRho = 1.0 - kRho;
-const double kK1 = 3.72, kK2 = 0.527, kK3 = 0.53;
+// k1..k3: 3T values by default (constants.h:86-89); orc_set_bw_friston2003(1)
+// switches to the BW_FRISTON2003 build's 7 rho, 2, 2 rho - 0.2 (:82-85)
+double kK1 = 3.72, kK2 = 0.527, kK3 = 0.53;
 
 inline double clip01(double v) { return std::max(0.0, std::min(v, 1.0)); }
 
@@ -164,6 +166,20 @@ int n_pairs_of(int nodes, int exc) {
 }  // namespace
 
 extern "C" {
+
+// Selects the BOLD weights for every later call (not thread-safe: set it
+// before running simulations).
+void orc_set_bw_friston2003(int on) {
+    if (on) {
+        kK1 = 7 * kRho;
+        kK2 = 2.0;
+        kK3 = 2 * kRho - 0.2;
+    } else {
+        kK1 = 3.72;
+        kK2 = 0.527;
+        kK3 = 0.53;
+    }
+}
 
 // ---- index tables ---------------------------------------------------------
 
@@ -457,6 +473,23 @@ int orc_fcd_tril(const double* bold, int n_vols, int N, int step, int window_siz
             fcd_out[idx++] = r;
         }
     return 0;
+}
+
+// The window FCs of orc_fcd_tril alone ([n_windows x n_pairs]; the windows
+// are independent, so they are spread over OpenMP threads).  Returns the
+// window count, or -1 if any correlation is NaN.
+int orc_window_fc(const double* bold, int n_vols, int N, int step, int window_size, int exc,
+                  double* window_fc) {
+    const int np = n_pairs_of(N, exc);
+    std::vector<int> st(n_vols + 1), en(n_vols + 1);
+    const int nw = orc_windows(n_vols, 0, window_size, step, st.data(), en.data());
+    int bad = 0;
+#pragma omp parallel for schedule(dynamic) reduction(|| : bad)
+    for (int w = 0; w < nw; ++w) {
+        const int len = en[w] - st[w] + 1;
+        if (orc_fc_tril(bold + (size_t)st[w] * N, len, N, exc, window_fc + (size_t)w * np)) bad = 1;
+    }
+    return bad ? -1 : nw;
 }
 
 // Two-sample Kolmogorov-Smirnov statistic D = sup_x |F_x(x) - F_y(x)|,

@@ -50,6 +50,9 @@ def lib():
         L.orc_windows.argtypes = [_i, _i, _i, _i, C.c_void_p, C.c_void_p]
         L.orc_noise.argtypes = [C.c_uint, _ll, _pf]
         L.orc_noise.restype = None
+        L.orc_window_fc.argtypes = [_pd, _i, _i, _i, _i, _i, _pd]
+        L.orc_set_bw_friston2003.argtypes = [_i]
+        L.orc_set_bw_friston2003.restype = None
         L.orc_mt19937_nth.argtypes = [C.c_uint, _ll]
         L.orc_mt19937_nth.restype = C.c_uint
         L.orc_regional_params.argtypes = [_i, _i, _pd, _d, _pd, _pd]
@@ -112,6 +115,12 @@ def noise(seed: int, count: int) -> np.ndarray:
     out = np.empty(count, np.float32)
     lib().orc_noise(seed, count, out)
     return out
+
+
+def set_bw_friston2003(on: bool) -> None:
+    """BOLD weights of the BW_FRISTON2003 build (constants.h:82-85) for every
+    later call, or back to the 3T defaults (:86-89)."""
+    lib().orc_set_bw_friston2003(int(on))
 
 
 def mt19937_nth(seed: int, n: int) -> int:
@@ -202,6 +211,16 @@ def fcd_tril(bold, step, window_size, exc, want_window_fc=False):
         return (None, None) if want_window_fc else None
     out = out[: nw * (nw - 1) // 2]
     return (out, wfc) if want_window_fc else out
+
+
+def window_fc(bold, step, window_size, exc):
+    """Window FCs [n_windows x n_pairs] of fcd_tril without the FCD itself."""
+    bold = np.ascontiguousarray(bold, np.float64)
+    T, N = bold.shape
+    st, _ = windows(T, 0, window_size, step)
+    out = np.zeros((len(st), n_pairs(N, exc)))
+    nw = lib().orc_window_fc(bold, T, N, step, window_size, int(exc), out)
+    return None if nw < 0 else out
 
 
 def ks(x, y) -> float:

@@ -290,3 +290,19 @@ def test_regional_params_closed_form(oracle_mod):
     out = oracle_mod.regional_params(maps, 0.1, [-2.0, 0.0])  # 0.1*(1-2) < 0.001 -> shifted
     raw = 0.1 * (1 - 2.0 * maps[0])
     np.testing.assert_allclose(out, raw - (raw.min() - 0.001), atol=1e-15)
+
+
+def test_window_fc_equals_fcd_tril_windows(oracle_mod):
+    """The window-FC-only entry (spread over threads) returns exactly the
+    window FCs that fcd_tril computes before its FCD (check_fit.cpp:64-136),
+    and each equals numpy's Pearson of that window."""
+    rng = np.random.default_rng(4)
+    bold = rng.normal(size=(57, 14))
+    for exc in (False, True):
+        w = oracle_mod.window_fc(bold, 3, 8, exc)
+        _, w2 = oracle_mod.fcd_tril(bold, 3, 8, exc, want_window_fc=True)
+        np.testing.assert_array_equal(w, w2)
+    st, en = oracle_mod.windows(57, 0, 8, 3)
+    c = np.corrcoef(bold[st[2]:en[2] + 1].T)
+    i, j = np.tril_indices(14, -1)
+    np.testing.assert_allclose(oracle_mod.window_fc(bold, 3, 8, False)[2], c[i, j], atol=1e-13)
