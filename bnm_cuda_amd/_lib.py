@@ -59,6 +59,7 @@ class Config(C.Structure):
         ("fic_reject_failed", C.c_int), ("analytical_fic", C.c_int), ("precision", C.c_int),
         ("fc_precision", C.c_int), ("kernel", C.c_int), ("sims_per_block", C.c_int),
         ("autotune", C.c_int), ("use_graph", C.c_int), ("save_hemis", C.c_int),
+        ("bw_friston2003", C.c_int),
     ]
 
 

@@ -91,8 +91,11 @@ struct bnm_ctx {
     size_t lds_max = 64 * 1024;  // dynamic LDS a workgroup may use
     float* noise = nullptr;
     long long noise_cap = 0;
+    // wIEfic holds the FIC starting wIE (the caller's or the analytical
+    // one) and is never written by the integration, so every run and graph
+    // replay starts from the same values; wIEadj receives the adjusted wIE
     double *G = nullptr, *wEE = nullptr, *wEI = nullptr, *wIE = nullptr, *wIEfic = nullptr,
-           *wIEscratch = nullptr;
+           *wIEadj = nullptr, *wIEscratch = nullptr;
     void* bold = nullptr;
     double* ext = nullptr;
     int *failed = nullptr, *trials = nullptr, *unstable = nullptr;
@@ -146,7 +149,7 @@ int bnm_default_config(bnm_config* c) {
     c->i_sampling_start = env_int("BNM_I_SAMPLING_START", 1000);
     c->i_sampling_end = env_int("BNM_I_SAMPLING_END", 10000);
     c->init_delta = (double)env_float("BNM_INIT_DELTA", 0.02);
-    c->noise_period_ms = 1000;
+    c->noise_period_ms = 0;   // finalize: the whole run, time_steps + 1 ms
     c->calculate_fic_penalty = 1;
     c->sim_only = 0;
     c->no_fcd = 0;
@@ -155,12 +158,15 @@ int bnm_default_config(bnm_config* c) {
     c->fic_penalty_scale = (double)env_float("BNM_CMAES_FIC_PENALTY_SCALE", 2.0);
     c->fic_reject_failed = env_int("BNM_CMAES_FIC_REJECT_FAILED", 0) != 0;
     c->analytical_fic = 1;
-    c->precision = BNM_PREC_F32;
-    c->fc_precision = BNM_FC_F32;
+    // the reference's default build computes in double (u_real = double
+    // unless USE_FLOATS, constants.h:8-22)
+    c->precision = BNM_PREC_F64;
+    c->fc_precision = BNM_FC_F64;
     c->kernel = BNM_KERNEL_AUTO;
     c->sims_per_block = 0;
     c->autotune = env_int("BNM_AUTOTUNE", 1) != 0;
     c->use_graph = 0;
+    c->bw_friston2003 = env_int("BNM_BW_FRISTON2003", 0) != 0;
     return BNM_OK;
 }
 
@@ -189,6 +195,7 @@ int bnm_finalize_config(bnm_config* c) {
     if (c->n_sims < 1) return fail(BNM_E_ARG, "n_sims < 1");
     if (c->bold_tr <= 0 || c->time_steps < 0) return fail(BNM_E_ARG, "bad time_steps / bold_tr");
     if (c->n_vols_remove < 0) c->n_vols_remove = env_int("BNM_BOLD_REMOVE", 30) * 1000 / c->bold_tr;
+    if (c->noise_period_ms == 0) c->noise_period_ms = (long long)c->time_steps + 1;
     if (c->noise_period_ms < 1) return fail(BNM_E_ARG, "noise_period_ms < 1");
     if (c->window_step < 1 || c->window_size < 1) return fail(BNM_E_ARG, "bad FCD window");
     if (c->i_sampling_end < c->i_sampling_start) return fail(BNM_E_ARG, "bad FIC sampling window");
@@ -274,7 +281,7 @@ int bnm_make_noise_chunked(unsigned seed, long long count, float* out, int threa
 void bnm_ctx_destroy(bnm_ctx* x) {
     if (!x) return;
     void* bufs[] = {x->SC64, x->SC32, x->SCt64, x->SCt32, x->SCc32, x->SCc64, x->noise, x->G, x->wEE, x->wEI, x->wIE, x->wIEfic,
-                    x->wIEscratch, x->bold, x->ext, x->failed, x->trials, x->unstable,
+                    x->wIEadj, x->wIEscratch, x->bold, x->ext, x->failed, x->trials, x->unstable,
                     x->seg_start, x->seg_len, x->pidx, x->mean, x->inv, x->wmean, x->winv, x->fc,
                     x->wfc, x->wz, x->wzlo, x->fcd, x->emp_c, x->ks_u, x->ks_fe, x->bins, x->nanf, x->corr, x->pen,
                     x->fcd_h[0], x->fcd_h[1], x->wmean_h[0], x->wmean_h[1], x->winv_h[0], x->winv_h[1]};
@@ -310,6 +317,7 @@ static int ctx_alloc(bnm_ctx* x) {
     if ((r = dalloc(&x->wEI, B * N))) return r;
     if ((r = dalloc(&x->wIE, B * N))) return r;
     if ((r = dalloc(&x->wIEfic, B * N))) return r;
+    if ((r = dalloc(&x->wIEadj, B * N))) return r;
     if ((r = dalloc(&x->wIEscratch, B * N))) return r;
     HIPCHK(hipMalloc(&x->bold, B * x->ots * N * rsz));
     if ((r = dalloc(&x->ext, B * 6 * N))) return r;
@@ -473,7 +481,10 @@ bnm_ctx* bnm_ctx_create(const bnm_config* cfg, int* status) {
     // with BNM_EI_LS=4); off by default: bit-identical to the LDS form and
     // slower at N = 100 (0.447 vs 0.382 us per substep)
     x->ei_bc = env_int("BNM_EI_BC", 0);
-    // BNM_EI_FORM pins the E/I update form (0 as written, 2 / 3 short chain)
+    // BNM_EI_FORM pins the E/I update form (0 as written, 2 / 3 short chain,
+    // 67 = 3 | 64 strict short chain, the default; 75 / 11: strict / plain
+    // with the FP64 BW step, 27: FP64 update, diagnostics of the N = 100
+    // shape)
     x->ei_form = env_int("BNM_EI_FORM", -1);
     // BNM_HYB_BC: 0 / 1 / 2 restricts the hybrid kernel to LDS / DPP-broadcast
     // S_E / DPP broadcast with three streamed chunks in flight
@@ -660,8 +671,9 @@ static std::string variant_name(const bnm_ctx* x, const Variant& v) {
     } else if (v.kind == 4) {
         const FastShape s = ei_shape(x->N, v.ls, v.mode == 1);
         static const char* fm[] = {"", ",dpp", ",short4", ",short1"};
-        std::snprintf(name, sizeof name, "split_ei_f32(LS=%d,NW=%d,KC=%d,SS=%d,threads=%d%s)", s.LS, s.NW, s.KC,
-                      s.SS, s.threads, fm[v.mode & 3]);
+        std::snprintf(name, sizeof name, "split_ei_f32(LS=%d,NW=%d,KC=%d,SS=%d,threads=%d%s%s%s%s)", s.LS, s.NW,
+                      s.KC, s.SS, s.threads, fm[v.mode & 3], v.mode & 64 ? ",strict" : "",
+                      v.mode & 8 ? ",bw64" : "", v.mode & 16 ? ",upd64" : "");
     } else {
         std::snprintf(name, sizeof name, "plain_%s(threads=%d)", p, ((x->N + 63) / 64) * 64);
     }
@@ -712,7 +724,7 @@ static std::vector<Variant> candidates(const bnm_ctx* x) {
     }
     if (x->ei_ls > 0) {
         if (x->f64) add({6, x->ei_ls, 1, 0});
-        else add({4, x->ei_ls, 1, x->ei_form >= 0 ? x->ei_form : x->ei_bc == 1 ? 1 : 0});
+        else add({4, x->ei_ls, 1, x->ei_form >= 0 ? x->ei_form : x->ei_bc == 1 ? 1 : (3 | 64)});
         if (!out.empty()) return out;
     }
     const bool pinned = x->fast_ls > 0 || x->fast_ks > 0;
@@ -727,11 +739,12 @@ static std::vector<Variant> candidates(const bnm_ctx* x) {
         }
         if (!x->f64 && !x->no_ei) {
             if (x->ei_bc == 1) add({4, 4, 1, 1});
-            // form 3 (short chain, one partial sum) is tuned by default; forms
-            // 2 and 0 (within ~1.5 % / 4 %, below the tuner's window noise)
-            // only when pinned
-            for (int f : {3, 2, 0})
-                if (x->ei_form < 0 ? f == 3 : x->ei_form == f) {
+            // form 3|64 (strict short chain, one partial sum) is tuned by
+            // default; the plain forms 3, 2 and 0 (faster by ~6 %, with a
+            // systematic per-TR BOLD error of up to ~2e-5 against FP64) only
+            // when pinned with BNM_EI_FORM
+            for (int f : {3 | 64, 3, 2, 0})
+                if (x->ei_form < 0 ? f == (3 | 64) : x->ei_form == f) {
                     add({4, 4, 1, f});
                     add({4, 2, 1, f});
                 }
@@ -782,11 +795,14 @@ static SimArgs make_args(bnm_ctx* x) {
     a.wEE = x->wEE;
     a.wEI = x->wEI;
     a.wIE_in = c.do_fic ? x->wIEfic : x->wIE;
-    a.wIE_out = c.do_fic ? x->wIEfic : x->wIEscratch;
+    a.wIE_out = c.do_fic ? x->wIEadj : x->wIEscratch;
     a.bold = x->bold;
     a.ext = x->ext;
     a.fic_failed = x->failed;
     a.fic_trials = x->trials;
+    a.bw_k1 = c.bw_friston2003 ? C64::k1_F : C64::k1;
+    a.bw_k2 = c.bw_friston2003 ? C64::k2_F : C64::k2;
+    a.bw_k3 = c.bw_friston2003 ? C64::k3_F : C64::k3;
     return a;
 }
 
@@ -1214,7 +1230,7 @@ int bnm_ctx_get_outputs(bnm_ctx* x, double* corr, double* pen, double* wIE_fic, 
     if (corr) HIPCHK(hipMemcpy(corr, x->corr, B * sizeof(double), hipMemcpyDeviceToHost));
     if (pen) HIPCHK(hipMemcpy(pen, x->pen, B * sizeof(double), hipMemcpyDeviceToHost));
     if (wIE_fic && x->cfg.do_fic)
-        HIPCHK(hipMemcpy(wIE_fic, x->wIEfic, B * N * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(wIE_fic, x->wIEadj, B * N * sizeof(double), hipMemcpyDeviceToHost));
     if (failed) HIPCHK(hipMemcpy(failed, x->failed, B * sizeof(int), hipMemcpyDeviceToHost));
     if (trials) HIPCHK(hipMemcpy(trials, x->trials, B * sizeof(int), hipMemcpyDeviceToHost));
     double* ext_out[6] = {S_E, I_E, r_E, S_I, I_I, r_I};

@@ -3,7 +3,7 @@
 // Restates the semantics of cmaes.cpp (CMAES_Points) of the reference: N.
 // Hansen's CMA-ES (2006 variant by default, 2016 with negative weights as
 // "variant 16"), sample points clipped into the unit box, the IQR-based
-// boundary penalty of Hansen et al. 2009 (Gamma / ksi terms added to the cost
+// boundary penalty of Hansen et al. 2009 (per-coordinate weight / scale terms added to the cost
 // of clipped points), early stop when the best value has not moved by more
 // than `early_stop_tol` over `early_stop_gens` generations.
 //
@@ -91,7 +91,7 @@ struct bnm_cmaes {
     unsigned seed_mw;
     double sigma, alphacov, gamma_scale, soft_edge, early_tol;
     double cc = 0, ccov = 0, csigma = 0, dsigma = 0, chiN = 0, muecov = 0, mueeff = 0, c1 = 0, cmue = 0;
-    std::vector<double> weights, Gamma, ksi, psigma, pc, m, malt, mpref, IQAhist, C, B, Dd;
+    std::vector<double> weights, bound_weight, bound_scale, psigma, pc, m, m_prev, m_feasible, iqr_history, C, B, Dd;
     std::vector<double> X, Xfeas, Xpref;   // [D][lambda]
     std::vector<double> pool;              // [n_pool][D], raw (unclipped) points
     int n_pool = 0;
@@ -157,13 +157,13 @@ bnm_cmaes* bnm_cmaes_create(int dim, int lambda, unsigned seed_mw, double sigma0
     e->ccov = (1 / e->muecov) * t1 + (1 - 1 / e->muecov) * t2;
     e->dsigma = 1 + 2 * std::max(0., std::sqrt((e->mueeff - 1) / (D + 1.)) - 1) + e->csigma;
     e->chiN = std::sqrt((double)D) * (1 - 1. / (4 * D) + 1. / (21. * D * D));
-    e->Gamma.assign(D, 0.0);
-    e->ksi.assign(D, 1.0);
+    e->bound_weight.assign(D, 0.0);
+    e->bound_scale.assign(D, 1.0);
     e->psigma.assign(D, 0.0);
     e->pc.assign(D, 0.0);
-    e->malt.assign(D, 0.0);
-    e->mpref.assign(D, 0.0);
-    e->IQAhist.assign(e->hist_len, 0.0);
+    e->m_prev.assign(D, 0.0);
+    e->m_feasible.assign(D, 0.0);
+    e->iqr_history.assign(e->hist_len, 0.0);
     e->C.assign((size_t)D * D, 0.0);
     e->B.assign((size_t)D * D, 0.0);
     e->Dd.assign(D, 1.0);
@@ -192,7 +192,7 @@ bnm_cmaes* bnm_cmaes_create(int dim, int lambda, unsigned seed_mw, double sigma0
                 q->Xpref[(size_t)i * q->lambda + j] =
                     std::min(1.0 - q->soft_edge, std::max(0.0 + q->soft_edge, v));
             }
-            q->mpref[i] = std::min(1.0 - q->soft_edge, std::max(0.0 + q->soft_edge, q->m[i]));
+            q->m_feasible[i] = std::min(1.0 - q->soft_edge, std::max(0.0 + q->soft_edge, q->m[i]));
         }
     };
     clip(e);
@@ -223,11 +223,11 @@ int bnm_cmaes_tell(bnm_cmaes* e, const double* cost, double* penalized) {
     {
         std::vector<double> sorted = raw;
         std::sort(sorted.begin(), sorted.end());
-        double IQA = quantile_sorted(sorted, 0.75) - quantile_sorted(sorted, 0.25);
+        double iqr_now = quantile_sorted(sorted, 0.75) - quantile_sorted(sorted, 0.25);
         double diag = 0;
         for (int i = 0; i < D; ++i) diag += e->C[(size_t)i * D + i];
-        IQA /= (e->sigma * e->sigma * diag);
-        e->IQAhist[(e->it - 1) % e->hist_len] = IQA;
+        iqr_now /= (e->sigma * e->sigma * diag);
+        e->iqr_history[(e->it - 1) % e->hist_len] = iqr_now;
         double logdiag = 0;
         for (int i = 0; i < D; ++i) {
             const double cii = e->C[(size_t)i * D + i];
@@ -235,21 +235,21 @@ int bnm_cmaes_tell(bnm_cmaes* e, const double* cost, double* penalized) {
             logdiag += std::log(cii);
             if (e->m[i] < e->soft_edge || e->m[i] > 1 - e->soft_edge || e->it == 1) {
                 const int nh = e->it < e->hist_len ? e->it + 1 : e->hist_len;
-                std::vector<double> h(e->IQAhist.begin(), e->IQAhist.begin() + std::min(nh, e->hist_len));
-                e->Gamma[i] = 5.0001 * D * median_of(h);
-                const double dm = e->m[i] - e->mpref[i];
+                std::vector<double> h(e->iqr_history.begin(), e->iqr_history.begin() + std::min(nh, e->hist_len));
+                e->bound_weight[i] = 5.0001 * D * median_of(h);
+                const double dm = e->m[i] - e->m_feasible[i];
                 if (std::fabs(dm) > 3 * e->sigma * std::sqrt(cii) * std::max(1.0, std::sqrt((double)D) / e->mueeff) &&
-                    ((dm < 0) == (e->m[i] - e->malt[i] < 0)))
-                    e->Gamma[i] *= 1.1;
-                e->Gamma[i] *= e->gamma_scale;
+                    ((dm < 0) == (e->m[i] - e->m_prev[i] < 0)))
+                    e->bound_weight[i] *= 1.1;
+                e->bound_weight[i] *= e->gamma_scale;
             }
         }
         for (int i = 0; i < D; ++i)
-            e->ksi[i] = std::exp(0.9 * (std::log(e->C[(size_t)i * D + i]) - logdiag / D));
+            e->bound_scale[i] = std::exp(0.9 * (std::log(e->C[(size_t)i * D + i]) - logdiag / D));
         for (int j = 0; j < L; ++j)
             for (int i = 0; i < D; ++i) {
                 const double d = e->Xpref[(size_t)i * L + j] - e->X[(size_t)i * L + j];
-                corr[j] += e->Gamma[i] / e->ksi[i] / D * d * d;
+                corr[j] += e->bound_weight[i] / e->bound_scale[i] / D * d * d;
             }
     }
     if (penalized) std::copy(corr.begin(), corr.end(), penalized);
@@ -262,17 +262,17 @@ int bnm_cmaes_tell(bnm_cmaes* e, const double* cost, double* penalized) {
         e->best_raw = raw[idx[0]];
         for (int i = 0; i < D; ++i) e->best_x[i] = e->Xfeas[(size_t)i * L + idx[0]];
     }
-    std::vector<double> BDz(D), tmp(D);
+    std::vector<double> mean_step(D), tmp(D);
     for (int i = 0; i < D; ++i) {
-        e->malt[i] = e->m[i];
+        e->m_prev[i] = e->m[i];
         double s = 0;
         for (int k = 0; k < mu; ++k) s += e->weights[k] * e->X[(size_t)i * L + idx[k]];
         e->m[i] = s;
-        BDz[i] = std::sqrt(e->mueeff) * (e->m[i] - e->malt[i]) / e->sigma;
+        mean_step[i] = std::sqrt(e->mueeff) * (e->m[i] - e->m_prev[i]) / e->sigma;
     }
     for (int i = 0; i < D; ++i) {
         double s = 0;
-        for (int j = 0; j < D; ++j) s += e->B[(size_t)j * D + i] * BDz[j];
+        for (int j = 0; j < D; ++j) s += e->B[(size_t)j * D + i] * mean_step[j];
         tmp[i] = s / e->Dd[i];
     }
     double np2 = 0;
@@ -285,7 +285,7 @@ int bnm_cmaes_tell(bnm_cmaes* e, const double* cost, double* penalized) {
     const double lhs = std::sqrt(np2) / std::sqrt(1 - std::pow(1 - e->csigma, 2 * e->it)) / e->chiN;
     const int hsig = e->variant == 16 ? (lhs < 1.4 + 2. / (D + 1)) : (lhs < 1.5 + 1. / (D - 0.5));
     for (int i = 0; i < D; ++i)
-        e->pc[i] = (1 - e->cc) * e->pc[i] + hsig * std::sqrt(e->cc * (2 - e->cc)) * BDz[i];
+        e->pc[i] = (1 - e->cc) * e->pc[i] + hsig * std::sqrt(e->cc * (2 - e->cc)) * mean_step[i];
     e->sigma *= std::exp((std::sqrt(np2) / e->chiN - 1) * e->csigma / e->dsigma);
     std::vector<double>& C = e->C;
     const double s2 = e->sigma * e->sigma;
@@ -296,7 +296,7 @@ int bnm_cmaes_tell(bnm_cmaes* e, const double* cost, double* penalized) {
                            (e->ccov / e->muecov) * (e->pc[i] * e->pc[j] + (1 - hsig) * e->cc * (2 - e->cc) * C[(size_t)i * D + j]);
                 for (int k = 0; k < mu; ++k)
                     v += e->ccov * (1 - 1 / e->muecov) * e->weights[k] *
-                         (e->X[(size_t)i * L + idx[k]] - e->malt[i]) * (e->X[(size_t)j * L + idx[k]] - e->malt[j]) / s2;
+                         (e->X[(size_t)i * L + idx[k]] - e->m_prev[i]) * (e->X[(size_t)j * L + idx[k]] - e->m_prev[j]) / s2;
                 C[(size_t)i * D + j] = C[(size_t)j * D + i] = v;
             }
     } else {
@@ -308,8 +308,8 @@ int bnm_cmaes_tell(bnm_cmaes* e, const double* cost, double* penalized) {
                 double v = (1 - e->c1 - e->cmue * wsum) * Cold[(size_t)i * D + j] +
                            e->c1 * (e->pc[i] * e->pc[j] + (1 - hsig) * e->cc * (2 - e->cc) * Cold[(size_t)i * D + j]);
                 for (int k = 0; k < mu; ++k)
-                    v += e->cmue * e->weights[k] * (e->X[(size_t)i * L + idx[k]] - e->malt[i]) *
-                         (e->X[(size_t)j * L + idx[k]] - e->malt[j]) / s2;
+                    v += e->cmue * e->weights[k] * (e->X[(size_t)i * L + idx[k]] - e->m_prev[i]) *
+                         (e->X[(size_t)j * L + idx[k]] - e->m_prev[j]) / s2;
                 for (int k = mu; k < L; ++k) {
                     std::vector<double> y(D), t(D), h(D);
                     for (int z = 0; z < D; ++z) y[z] = (e->X[(size_t)z * L + idx[k]] - e->m[z]) / e->sigma;
@@ -324,8 +324,8 @@ int bnm_cmaes_tell(bnm_cmaes* e, const double* cost, double* penalized) {
                         for (int zz = 0; zz < D; ++zz) s += e->B[(size_t)z * D + zz] * t[zz];
                         nh += s * s;
                     }
-                    v += e->cmue * e->weights[k] * (D / nh) * (e->X[(size_t)i * L + idx[k]] - e->malt[i]) *
-                         (e->X[(size_t)j * L + idx[k]] - e->malt[j]) / s2;
+                    v += e->cmue * e->weights[k] * (D / nh) * (e->X[(size_t)i * L + idx[k]] - e->m_prev[i]) *
+                         (e->X[(size_t)j * L + idx[k]] - e->m_prev[j]) / s2;
                 }
                 C[(size_t)i * D + j] = C[(size_t)j * D + i] = v;
             }
@@ -354,7 +354,7 @@ int bnm_cmaes_tell(bnm_cmaes* e, const double* cost, double* penalized) {
             e->Xfeas[(size_t)i * L + j] = std::min(1.0, std::max(0.0, v));
             e->Xpref[(size_t)i * L + j] = std::min(1.0 - e->soft_edge, std::max(e->soft_edge, v));
         }
-        e->mpref[i] = std::min(1.0 - e->soft_edge, std::max(e->soft_edge, e->m[i]));
+        e->m_feasible[i] = std::min(1.0 - e->soft_edge, std::max(e->soft_edge, e->m[i]));
     }
     e->tvec[e->it % e->early_gens] = e->best_val;
     if (e->it > e->early_gens) {

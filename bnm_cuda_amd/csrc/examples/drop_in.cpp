@@ -2,9 +2,12 @@
 // (include/bnm_run_simulations.hpp): one generation of a small population on
 // the GPU, printing the cost, FIC penalty and FIC trial count per particle.
 // Usage: drop_in <nodes> <n_sims> <time_steps_ms> [precision: 0=f64, 1=f32]
+//                [sims_out_prefix: with n_sims = 1, save_output writes the
+//                 reference's bold.txt, S_E.txt ... fic_ntrials.txt there]
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "../../../include/bnm_run_simulations.hpp"
@@ -13,7 +16,9 @@ int main(int argc, char** argv) {
     const int N = argc > 1 ? std::atoi(argv[1]) : 20;
     const int B = argc > 2 ? std::atoi(argv[2]) : 4;
     const int T = argc > 3 ? std::atoi(argv[3]) : 60000;
-    const int prec = argc > 4 ? std::atoi(argv[4]) : BNM_PREC_F32;
+    const int prec = argc > 4 ? std::atoi(argv[4]) : BNM_PREC_F64;
+    const std::string prefix = argc > 5 ? argv[5] : "";
+    const bool save = !prefix.empty();
     const int TR = 1000, ws = 10, step = 2, nvr = 10;
     // ring SC with unit row sums
     std::vector<double> SC((size_t)N * N, 0.0);
@@ -52,7 +57,7 @@ int main(int argc, char** argv) {
     opt.fc_precision = prec == BNM_PREC_F64 ? BNM_FC_F64 : BNM_FC_F32;
     const int st = bnm::run_simulations(
         corr.data(), pen.data(), G.data(), wEE.data(), wEI.data(), wIE.data(), wfic.data(),
-        SC.data(), N, true, T, TR, 10, false, emp_fc, emp_fcd, ws, true, B, noise.data(), nlen,
+        SC.data(), N, true, T, TR, 10, false, emp_fc, emp_fcd, ws, save, true, prefix, B, noise.data(), nlen,
         bold.data(), nullptr, nullptr, rE.data(), nullptr, nullptr, nullptr, true, true,
         failed.data(), trials.data(), nvr, fc.data(), fcd.data(), opt);
     if (st) {

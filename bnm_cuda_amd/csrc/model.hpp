@@ -34,7 +34,10 @@ struct C64 {
     static constexpr double y = 1.0 / 0.41, kappa = 1.0 / 0.65;
     static constexpr double V0 = 0.02 * 100;
     static constexpr double ialpha = 1.0 / 0.32, itau = 1.0 / 0.98, oneminrho = 1.0 - 0.34;
+    // BOLD weights: 3T values of Demirtas 2019 (the reference's default) and,
+    // with BW_FRISTON2003, Friston 2003's 7 rho, 2, 2 rho - 0.2
     static constexpr double k1 = 3.72, k2 = 0.527, k3 = 0.53;
+    static constexpr double k1_F = 7 * 0.34, k2_F = 2.0, k3_F = 2 * 0.34 - 0.2;
     // FIC steady state of the isolated node (constants.h:57-68)
     static constexpr double tau_I_s = 0.01, gamma_I_s = 1.0;
     static constexpr double I_E_ss = 0.3773805650, S_E_ss = 0.1647572075;
@@ -123,39 +126,82 @@ template <> struct BW<double> {
                      pow(nu, C64::ialpha) * q / nu);
         f = f_new;
     }
-    __device__ __forceinline__ double bold() const {
+    __device__ __forceinline__ double bold(double k1, double k2, double k3) const {
 #pragma clang fp contract(off)
-        return C64::V0 * (C64::k1 * (1 - q) + C64::k2 * (1 - q / nu) + C64::k3 * (1 - nu));
+        return C64::V0 * (k1 * (1 - q) + k2 * (1 - q / nu) + k3 * (1 - nu));
     }
 };
 
-// FP32 flavour: the BW state stays in FP64 and only the 1 ms increments are
-// formed in FP32 (v_exp_f32 / v_log_f32 / v_rcp_f32).  f, nu and q sit near
-// 1 and change by ~1e-4 per step, so FP32 state would lose ~6e-8 per step to
-// rounding; FP64 state keeps the accumulated BOLD error at the increment's
-// relative precision.
+// FP32 flavour.  f, nu and q sit near 1 and BOLD is formed from 1 - q,
+// 1 - q/nu and 1 - nu, so the step works on the deviations f - 1, nu - 1,
+// q - 1 (kept in FP64; the increments are formed in FP32):
+//   nu^(1/alpha) - 1 = expm1(ln(1 + u) / alpha),        u = nu - 1
+//   (1-rho)^(1/f) = (1-rho) (1 + expm1(-ln(1-rho) (f-1)/f))
+//   f (1 - (1-rho)^(1/f)) / rho - 1 = (f-1) - f (1-rho)/rho expm1(..)
+//   nu^(1/alpha) q / nu - 1 = (E + (q-1) + E (q-1) - u) / (1 + u)
+// with ln(1+u) = 2 atanh(u / (2+u)) (odd series) and expm1 as a degree-9
+// Taylor polynomial: each term then carries FP32 relative error of a small
+// quantity instead of the ~1e-7 absolute error of v_log_f32 / v_exp_f32 near
+// 1, which the 1 - q cancellation in BOLD would turn into ~1e-6 relative.
+// nu^(1/alpha) at the updated nu (the q step uses the new nu, bnm.cpp:209)
+// follows by a first-order step from the old one (|dnu| ~ 1e-6: the
+// second-order term is ~1e-12).  Arguments outside the series' range fall
+// back to the hardware functions.
+__device__ __forceinline__ float bw_expm1(float z) {
+    if (fabsf(z) > 0.5f) return __builtin_amdgcn_exp2f(z * (float)kLog2e) - 1.0f;
+    float p = 1.0f / 362880.0f;
+    p = fmaf(p, z, 1.0f / 40320.0f);
+    p = fmaf(p, z, 1.0f / 5040.0f);
+    p = fmaf(p, z, 1.0f / 720.0f);
+    p = fmaf(p, z, 1.0f / 120.0f);
+    p = fmaf(p, z, 1.0f / 24.0f);
+    p = fmaf(p, z, 1.0f / 6.0f);
+    p = fmaf(p, z, 0.5f);
+    p = fmaf(p, z, 1.0f);
+    return p * z;
+}
+
+__device__ __forceinline__ float bw_log1p(float u) {
+    const float s = u * __builtin_amdgcn_rcpf(2.0f + u);
+    if (fabsf(s) > 0.15f) return __builtin_amdgcn_logf(1.0f + u) * (float)(1.0 / kLog2e);
+    const float s2 = s * s;
+    float p = 1.0f / 11.0f;
+    p = fmaf(p, s2, 1.0f / 9.0f);
+    p = fmaf(p, s2, 1.0f / 7.0f);
+    p = fmaf(p, s2, 1.0f / 5.0f);
+    p = fmaf(p, s2, 1.0f / 3.0f);
+    p = fmaf(p, s2, 1.0f);
+    return 2.0f * s * p;
+}
+
 template <> struct BW<float> {
-    double x = 0.0, f = 1.0, nu = 1.0, q = 1.0;
-    __device__ __forceinline__ void reset() { x = 0.0; f = 1.0; nu = 1.0; q = 1.0; }
+    double x = 0.0, fm = 0.0, um = 0.0, qm = 0.0;   // x, f - 1, nu - 1, q - 1
+    __device__ __forceinline__ void reset() { x = 0.0; fm = 0.0; um = 0.0; qm = 0.0; }
     __device__ __forceinline__ void step(float SE) {
         constexpr float mdt = (float)C64::model_dt;
         constexpr float mdt_itau = (float)(C64::model_dt * C64::itau);
         constexpr float kap = (float)C64::kappa, yy = (float)C64::y;
-        constexpr float ia = (float)C64::ialpha, irho = (float)(1.0 / C64::rho);
-        const float xf = (float)x, fm1 = (float)(f - 1.0), ff = (float)f;
-        x += (double)(mdt * (SE - kap * xf - yy * fm1));
-        const double f_new = f + (double)(mdt * (float)x);
-        const float nu_ia = __builtin_amdgcn_exp2f(ia * __builtin_amdgcn_logf((float)nu));
-        nu += (double)(mdt_itau * (ff - nu_ia));
-        const float nuf = (float)nu;
-        const float nu_ia_new = __builtin_amdgcn_exp2f(ia * __builtin_amdgcn_logf(nuf));
-        const float p = __builtin_amdgcn_exp2f((float)kLog2OneMinRho * __builtin_amdgcn_rcpf(ff));
-        q += (double)(mdt_itau * (ff * (1.0f - p) * irho -
-                                  nu_ia_new * (float)q * __builtin_amdgcn_rcpf(nuf)));
-        f = f_new;
+        constexpr float ia = (float)C64::ialpha;
+        constexpr float c1 = (float)(C64::oneminrho / C64::rho);
+        constexpr float c2 = (float)(-kLog2OneMinRho / kLog2e);   // -ln(1 - rho)
+        const float xf = (float)x, f1 = (float)fm, u = (float)um, q1 = (float)qm;
+        x += (double)(mdt * (SE - kap * xf - yy * f1));
+        const double fm_new = fm + (double)(mdt * (float)x);
+        const float ru = __builtin_amdgcn_rcpf(1.0f + u);
+        const float E = bw_expm1(ia * bw_log1p(u));          // nu^(1/alpha) - 1
+        const float dnu = mdt_itau * (f1 - E);
+        um += (double)dnu;
+        const float un = (float)um;
+        const float En = fmaf(ia * (1.0f + E) * ru, dnu, E);  // at the new nu
+        const float em = bw_expm1(c2 * f1 * __builtin_amdgcn_rcpf(1.0f + f1));
+        const float t1 = f1 - (1.0f + f1) * c1 * em;
+        const float t2 = (En + q1 + En * q1 - un) * __builtin_amdgcn_rcpf(1.0f + un);
+        qm += (double)(mdt_itau * (t1 - t2));
+        fm = fm_new;
     }
-    __device__ __forceinline__ float bold() const {
-        return (float)(C64::V0 * (C64::k1 * (1 - q) + C64::k2 * (1 - q / nu) + C64::k3 * (1 - nu)));
+    __device__ __forceinline__ float bold(double k1, double k2, double k3) const {
+        // V0 (k1 (1 - q) + k2 (1 - q/nu) + k3 (1 - nu)), 1 - q/nu = (u - q1)/(1 + u)
+        return (float)(C64::V0 * (-k1 * qm + k2 * (um - qm) / (1.0 + um) - k3 * um));
     }
 };
 
@@ -206,6 +252,7 @@ struct SimArgs {
     double* ext;              // [B][6][N] means of S_E, I_E, r_E, S_I, I_I, r_I
     int* fic_failed;          // [B]
     int* fic_trials;          // [B]
+    double bw_k1, bw_k2, bw_k3;   // BOLD weights (C64::k1..k3 or k1_F..k3_F)
 };
 
 }  // namespace bnm

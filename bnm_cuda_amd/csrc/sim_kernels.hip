@@ -59,7 +59,7 @@ __device__ __forceinline__ void load_params(const SimArgs& a, int sim, int j, La
 template <class R>
 __device__ __forceinline__ void sample(const SimArgs& a, int sim, int j, int vol, Lane<R>& L) {
     R* bold = reinterpret_cast<R*>(a.bold);
-    bold[((size_t)sim * a.output_ts + vol) * a.N + j] = L.bw.bold();
+    bold[((size_t)sim * a.output_ts + vol) * a.N + j] = L.bw.bold(a.bw_k1, a.bw_k2, a.bw_k3);
     if (vol >= a.n_vols_remove) {
         L.ext[0] += (double)L.SE;
         L.ext[1] += (double)L.IE;
@@ -373,6 +373,19 @@ __device__ __forceinline__ void bcast_dot(float (&acc)[4], float src, const floa
     if constexpr (NQ >= 16) bcast_fma4<3>(acc, src, sc[Q0 + 12], sc[Q0 + 13], sc[Q0 + 14], sc[Q0 + 15]);
 }
 
+// FP64 node update of one pool (the FP64 E/I kernel's step, below).
+__device__ __forceinline__ void ei_step_f64(double c, double Gl, double k0, double kSE, double kSI, double aa,
+                                            double bb, double dd, double g, double m, double tau, double xi,
+                                            double SEv, double SIv, double& S, double& I, double& r) {
+#pragma clang fp contract(off)
+    I = k0 + kSE * SEv + c * Gl * C64::J_NMDA - kSI * SIv;
+    const double x = aa * I - bb;
+    const double e = exp(-dd * x);
+    r = x / (1 - e);
+    const double dS = xi * C64::sigma * C64::sqrt_dt + C64::dt * ((1 - m * S) * g * r - (S / tau));
+    S = fmax(0.0, fmin(S + dS, 1.0));
+}
+
 // FORM: 0 the update as written above; 1 the same with the DPP row broadcast;
 // 2 / 3 the short-chain update: everything that does not depend on the
 // coupling c is formed before it arrives, so that after the reduction
@@ -382,9 +395,30 @@ __device__ __forceinline__ void bcast_dot(float (&acc)[4], float src, const floa
 // rounding), I_E is formed only on the last substep of a millisecond (its
 // only use), and S_E / S_I come straight from the two halves of the lane
 // swap.  FORM 2 keeps four partial sums in the dot, FORM 3 one chain.
+// Precision bits on top of the form (FORM & 3):
+//   64 strict (short-chain forms): every coefficient that multiplies a state
+//      value (wEE / wEI, wIE, a*G*J_NMDA, w*I_0, dt*gamma, dt/tau,
+//      sigma*sqrt_dt, -d*log2(e)) is an FP32 hi + lo pair formed from its FP64
+//      value; the exponent is taken from the rounded x (y = -d*log2(e)*x,
+//      where the plain short chain forms it separately from rounded
+//      products); S is kept as an unevaluated sum hi + lo (the increment is
+//      added to lo and renormalised with Fast2Sum, starting from the
+//      reference's exact 0.001), and wIE / delta of the FIC stay in FP64.  A
+//      rounded coefficient shifts the fixed point by its relative rounding
+//      (~3e-8) times the loop gain, a systematic error that does not average
+//      out; the splits remove it at a few FMAs per substep.  What is left is
+//      the FP32 rounding of x, exp2 and the reciprocal in the rate.
+//   Diagnostics (N = 100 build only), measuring single error terms:
+//   8  Balloon-Windkessel step and BOLD in FP64 (libm pow)
+//   16 node update (currents, rate, S) in FP64 with only the coupling dot in
+//      FP32 (implies 8)
 template <int KC, int LS, int MAXT, int FORM>
 __global__ __launch_bounds__(MAXT) void k_sim_ei(const float* __restrict__ SC, SimArgs a, int NW, int SS) {
-    constexpr bool BC = FORM == 1, LEAN = FORM >= 2;
+    constexpr int F = FORM & 3;
+    constexpr bool STRICT = (FORM & 64) != 0, BW64 = (FORM & 8) != 0, UPD64 = (FORM & 16) != 0;
+    constexpr bool BC = F == 1, LEAN = F >= 2;
+    static_assert(!UPD64 || LEAN, "FP64 update in the short-chain forms only");
+    static_assert(!STRICT || LEAN, "strict coefficients in the short-chain forms only");
     static_assert(LS == 2 || LS == 4, "E/I split needs two lanes per node");
     static_assert(KC % 4 == 0, "slice length must be a multiple of 4");
     static_assert(!BC || (LS == 4 && KC <= 32), "row broadcast: 16-lane slices of at most 32 values");
@@ -424,6 +458,8 @@ __global__ __launch_bounds__(MAXT) void k_sim_ei(const float* __restrict__ SC, S
     const float bb = pop ? (float)C64::b_I : (float)C64::b_E;
     const float ndl = pop ? (float)(-C64::d_I * kLog2e) : (float)(-C64::d_E * kLog2e);
     const float invd = pop ? (float)(1.0 / C64::d_I) : (float)(1.0 / C64::d_E);
+    const double ndl64_ = pop ? -C64::d_I * kLog2e : -C64::d_E * kLog2e;
+    const float ndlh = (float)ndl64_, ndll = (float)(ndl64_ - (double)(float)ndl64_);
     const float d12 = pop ? (float)(C64::d_I / 12.0) : (float)(C64::d_E / 12.0);
     const float g = pop ? (float)(C64::dt * C64::gamma_I) : (float)(C64::dt * C64::gamma_E);
     const float gm = pop ? 0.0f : g;
@@ -435,9 +471,36 @@ __global__ __launch_bounds__(MAXT) void k_sim_ei(const float* __restrict__ SC, S
     const double kc64 = pop ? 0.0 : (double)GJ;
     const float aakc = (float)(aa64 * kc64), ndlaakc = (float)(ndl64 * aa64 * kc64);
     const float ndlaa = (float)(ndl64 * aa64), ndlbb = (float)(ndl64 * bb64);
+    // strict form: lo parts of the coefficients (value - (double)hi)
+    const double kSE64 = comp ? (pop ? a.wEI[o] : a.wEE[o]) : 0.0;
+    const float kSEl = (float)(kSE64 - (double)kSE);
+    const float k0l = (float)((pop ? C64::wI_I0 : C64::wE_I0) - (double)k0);
+    const double aakc64 = aa64 * (pop ? 0.0 : a.G[sim] * C64::J_NMDA);
+    const float aakch = (float)aakc64, aakcl = (float)(aakc64 - (double)(float)aakc64);
+    const double g64 = pop ? C64::dt * C64::gamma_I : C64::dt * C64::gamma_E;
+    const double it64 = pop ? C64::dt / C64::tau_I : C64::dt / C64::tau_E;
+    const float gl = (float)(g64 - (double)g), gml = pop ? 0.0f : gl;
+    const float itl = (float)(it64 - (double)it);
+    constexpr double nzs64 = C64::sigma * C64::sqrt_dt;
+    constexpr float nzsl = (float)(nzs64 - (double)(float)nzs64);
+    float wIEh = comp ? (float)a.wIE_in[o] : 0.0f;
+    float wIEl = comp ? (float)(a.wIE_in[o] - (double)wIEh) : 0.0f;
     float wIE = comp ? (float)a.wIE_in[o] : 0.0f, delta = (float)a.init_delta;
     float S = 0.001f, I = 0.0f, r = 0.0f;
+    // low part of S (strict): the reference starts from S = 0.001 exactly,
+    // float(0.001) is 4.7e-11 above it
+    constexpr float kS0l = (float)(0.001 - (double)0.001f);
+    float Sl = STRICT ? kS0l : 0.0f;
+    double Sd = 0.001, Id = 0.0, rd = 0.0;   // FP64 node state (bit 16)
+    double wIEd = comp ? a.wIE_in[o] : 0.0, deltad = a.init_delta;
+    const double Gd = pop ? 0.0 : a.G[sim];
+    const double k0d = pop ? C64::wI_I0 : C64::wE_I0;
+    const double kSEd = comp ? (pop ? a.wEI[o] : a.wEE[o]) : 0.0;
+    const double ddd = pop ? C64::d_I : C64::d_E;
+    const double gd = pop ? C64::gamma_I : C64::gamma_E, md = pop ? 0.0 : 1.0;
+    const double taud = pop ? C64::tau_I : C64::tau_E;
     BW<float> bw;
+    BW<double> bw64;
     double mI = 0, ext[3] = {0, 0, 0};
 
     int ts = 0, vol = 0, trial = 0, buf = 0;
@@ -457,7 +520,8 @@ __global__ __launch_bounds__(MAXT) void k_sim_ei(const float* __restrict__ SC, S
 #pragma unroll
             for (int s = 0; s < 10; ++s) nzn[s] = p[(size_t)s * 2 * N];
         }
-        const float kSI = pop ? 1.0f : wIE;
+        const float kSI = pop ? 1.0f : STRICT ? wIEh : wIE;
+        const float kSIl = pop ? 0.0f : wIEl;
 #pragma unroll
         for (int s = 0; s < 10; ++s) {
             const float4* Sr = reinterpret_cast<const float4*>(Sb + buf * ldS + sl * SS);
@@ -489,8 +553,16 @@ __global__ __launch_bounds__(MAXT) void k_sim_ei(const float* __restrict__ SC, S
                 const float Y0 = fmaf(ndlaa, pre, -ndlbb);
                 const float Ac = fmaf(-gm, S, g);
                 const float SB = fmaf(-it, S, fmaf(nz[s], nzs, S));
+                // strict form: the lo parts, all formed before c arrives
+                float X0s = 0.0f, Acl = 0.0f, D = 0.0f, prel = 0.0f;
+                if constexpr (STRICT) {
+                    prel = fmaf(-kSIl, __uint_as_float(w[1]), fmaf(kSEl, __uint_as_float(w[0]), k0l));
+                    X0s = fmaf(aa, prel, fmaf(aa, pre, -bb));
+                    Acl = fmaf(-gml, S, gl);
+                    D = fmaf(nz[s], nzsl, fmaf(-itl, S, fmaf(-it, S, nz[s] * nzs))) + Sl;
+                }
                 float c;
-                if constexpr (FORM == 3) {
+                if constexpr (F == 3) {
                     c = sc[0] * sv[0].x;
                     c = fmaf(sc[1], sv[0].y, c);
                     c = fmaf(sc[2], sv[0].z, c);
@@ -515,13 +587,45 @@ __global__ __launch_bounds__(MAXT) void k_sim_ei(const float* __restrict__ SC, S
                 }
                 if constexpr (LS == 4) c = lane_sum16<float>(c);
                 c = lane_sum32<float>(c);
-                const float x = fmaf(aakc, c, X0);
-                const float e = __builtin_amdgcn_exp2f(fmaf(ndlaakc, c, Y0));
-                const float direct = x * __builtin_amdgcn_rcpf(1.0f - e);
-                const float series = invd + x * (0.5f + d12 * x);
-                r = fabsf(x) < 0.1f ? series : direct;
-                S = __builtin_amdgcn_fmed3f(fmaf(r, Ac, SB), 0.0f, 1.0f);
-                if (s == 9) I = fmaf(kc, c, pre);
+                if constexpr (UPD64) {
+                    double SEv, SIv;
+                    {
+                        const unsigned long long u = __double_as_longlong(Sd);
+                        const unsigned lo = (unsigned)u, hi = (unsigned)(u >> 32);
+                        auto rl = LS == 4 ? __builtin_amdgcn_permlane16_swap(lo, lo, false, false)
+                                          : __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+                        auto rh = LS == 4 ? __builtin_amdgcn_permlane16_swap(hi, hi, false, false)
+                                          : __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+                        SEv = __longlong_as_double(((unsigned long long)rh[0] << 32) | rl[0]);
+                        SIv = __longlong_as_double(((unsigned long long)rh[1] << 32) | rl[1]);
+                    }
+                    ei_step_f64((double)c, Gd, k0d, kSEd, pop ? 1.0 : wIEd, aa64, bb64, ddd, gd, md, taud,
+                                (double)nz[s], SEv, SIv, Sd, Id, rd);
+                    S = (float)Sd;
+                    r = (float)rd;
+                    if (s == 9) I = (float)Id;
+                } else if constexpr (STRICT) {
+                    const float x = fmaf(aakch, c, fmaf(aakcl, c, X0s));
+                    const float y = fmaf(ndlh, x, ndll * x);
+                    const float direct = x * __builtin_amdgcn_rcpf(1.0f - __builtin_amdgcn_exp2f(y));
+                    const float series = invd + x * (0.5f + d12 * x);
+                    r = fabsf(x) < 0.1f ? series : direct;
+                    // S + lo + increment, the rounding of the sum carried in lo
+                    const float u = fmaf(r, Ac, fmaf(r, Acl, D));
+                    const float t = S + u;
+                    const float tc = __builtin_amdgcn_fmed3f(t, 0.0f, 1.0f);
+                    Sl = tc == t ? u - (t - S) : 0.0f;
+                    S = tc;
+                    if (s == 9) I = fmaf(kc, c, pre) + prel;
+                } else {
+                    const float x = fmaf(aakc, c, X0);
+                    const float e = __builtin_amdgcn_exp2f(fmaf(ndlaakc, c, Y0));
+                    const float direct = x * __builtin_amdgcn_rcpf(1.0f - e);
+                    const float series = invd + x * (0.5f + d12 * x);
+                    r = fabsf(x) < 0.1f ? series : direct;
+                    S = __builtin_amdgcn_fmed3f(fmaf(r, Ac, SB), 0.0f, 1.0f);
+                    if (s == 9) I = fmaf(kc, c, pre);
+                }
                 if (writer) Sb[(buf ^ 1) * ldS + posj] = S;
                 __syncthreads();
                 buf ^= 1;
@@ -557,30 +661,48 @@ __global__ __launch_bounds__(MAXT) void k_sim_ei(const float* __restrict__ SC, S
             __syncthreads();
             buf ^= 1;
         }
-        if (comp && pop == 0) bw.step(S);
+        if (comp && pop == 0) {
+            if constexpr (UPD64) bw64.step(Sd);
+            else if constexpr (BW64) bw64.step((double)S + (double)Sl);
+            else bw.step(S);
+        }
         if (ts % a.bold_tr == 0) {
             if (writer) {
                 float* bold = reinterpret_cast<float*>(a.bold);
-                bold[((size_t)sim * a.output_ts + vol) * N + j] = bw.bold();
+                bold[((size_t)sim * a.output_ts + vol) * N + j] = BW64 || UPD64 ? (float)bw64.bold(a.bw_k1, a.bw_k2, a.bw_k3) : bw.bold(a.bw_k1, a.bw_k2, a.bw_k3);
             }
             if (vol >= a.n_vols_remove) {
-                ext[0] += (double)S;
-                ext[1] += (double)I;
-                ext[2] += (double)r;
+                ext[0] += UPD64 ? Sd : (double)S + (double)Sl;
+                ext[1] += UPD64 ? Id : (double)I;
+                ext[2] += UPD64 ? rd : (double)r;
             }
             ++vol;
         }
         ++ts;
         bool restarted = false;
         if (adjusting) {
-            if (ts >= a.i_start && ts <= a.i_end) mI += (double)I;
+            if (ts >= a.i_start && ts <= a.i_end) mI += UPD64 ? Id : (double)I;
             if (ts == a.i_end) {
                 // the excitatory lanes of a node decide alike; inhibitory lanes never do
-                const bool out = comp && pop == 0 && fic_decide<float>(mI, dur, trial, a.max_fic_trials, wIE, delta);
+                bool out = false;
+                if (comp && pop == 0) {
+                    if constexpr (UPD64) {
+                        out = fic_decide<double>(mI, dur, trial, a.max_fic_trials, wIEd, deltad);
+                    } else if constexpr (STRICT) {
+                        out = fic_decide<double>(mI, dur, trial, a.max_fic_trials, wIEd, deltad);
+                        wIEh = (float)wIEd;
+                        wIEl = (float)(wIEd - (double)wIEh);
+                    } else {
+                        out = fic_decide<float>(mI, dur, trial, a.max_fic_trials, wIE, delta);
+                    }
+                }
                 if (__syncthreads_or(out)) {
                     if (trial < a.max_fic_trials) {
                         S = 0.001f;
+                        Sl = STRICT ? kS0l : 0.0f;
+                        Sd = 0.001;
                         bw.reset();
+                        bw64.reset();
                         mI = 0;
                         ext[0] = ext[1] = ext[2] = 0;
                         if (writer) Sb[buf * ldS + posj] = 0.001f;
@@ -606,7 +728,7 @@ __global__ __launch_bounds__(MAXT) void k_sim_ei(const float* __restrict__ SC, S
         }
     }
     if (owner) {
-        if (pop == 0) a.wIE_out[o] = (double)wIE;
+        if (pop == 0) a.wIE_out[o] = UPD64 || STRICT ? wIEd : (double)wIE;
         const int n = vol - a.n_vols_remove;
         for (int i = 0; i < 3; ++i)
             a.ext[((size_t)sim * 6 + 3 * pop + i) * N + j] = n > 0 ? ext[i] / n : 0.0;
@@ -630,17 +752,6 @@ __global__ __launch_bounds__(MAXT) void k_sim_ei(const float* __restrict__ SC, S
 // split kernel with the same LS, so the results are bit-identical to it; each
 // lane evaluates one FP64 exp per substep instead of two.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void ei_step_f64(double c, double Gl, double k0, double kSE, double kSI, double aa,
-                                            double bb, double dd, double g, double m, double tau, double xi,
-                                            double SEv, double SIv, double& S, double& I, double& r) {
-#pragma clang fp contract(off)
-    I = k0 + kSE * SEv + c * Gl * C64::J_NMDA - kSI * SIv;
-    const double x = aa * I - bb;
-    const double e = exp(-dd * x);
-    r = x / (1 - e);
-    const double dS = xi * C64::sigma * C64::sqrt_dt + C64::dt * ((1 - m * S) * g * r - (S / tau));
-    S = fmax(0.0, fmin(S + dS, 1.0));
-}
 
 template <int KC, int LS, int MAXT>
 __global__ __launch_bounds__(MAXT) void k_sim_ei64(const double* __restrict__ SC, SimArgs a, int NW, int SS) {
@@ -738,7 +849,7 @@ __global__ __launch_bounds__(MAXT) void k_sim_ei64(const double* __restrict__ SC
         if (ts % a.bold_tr == 0) {
             if (writer) {
                 double* bold = reinterpret_cast<double*>(a.bold);
-                bold[((size_t)sim * a.output_ts + vol) * N + j] = bw.bold();
+                bold[((size_t)sim * a.output_ts + vol) * N + j] = bw.bold(a.bw_k1, a.bw_k2, a.bw_k3);
             }
             if (vol >= a.n_vols_remove) {
                 ext[0] += S;
@@ -1373,19 +1484,33 @@ static void launch_ei(const float* SC, const SimArgs& a, const FastShape& s, hip
 }
 
 template <int KC, int LS>
-static void launch_ei_form(const float* SC, const SimArgs& a, const FastShape& s, int form, hipStream_t st) {
-    if (form == 2) launch_ei<KC, LS, 2>(SC, a, s, st);
-    else if (form == 3) launch_ei<KC, LS, 3>(SC, a, s, st);
-    else launch_ei<KC, LS, 0>(SC, a, s, st);
+static hipError_t launch_ei_form(const float* SC, const SimArgs& a, const FastShape& s, int form, hipStream_t st) {
+    switch (form) {
+        case 0: launch_ei<KC, LS, 0>(SC, a, s, st); return hipSuccess;
+        case 2: launch_ei<KC, LS, 2>(SC, a, s, st); return hipSuccess;
+        case 3: launch_ei<KC, LS, 3>(SC, a, s, st); return hipSuccess;
+        case 3 | 64: launch_ei<KC, LS, 3 | 64>(SC, a, s, st); return hipSuccess;
+        default: break;
+    }
+    // precision diagnostics, N = 100 shape only
+    if constexpr (KC == 28 && LS == 4) {
+        switch (form) {
+            case 3 | 64 | 8: launch_ei<KC, LS, 3 | 64 | 8>(SC, a, s, st); return hipSuccess;
+            case 3 | 8 | 16: launch_ei<KC, LS, 3 | 8 | 16>(SC, a, s, st); return hipSuccess;
+            case 3 | 8: launch_ei<KC, LS, 3 | 8>(SC, a, s, st); return hipSuccess;
+            default: break;
+        }
+    }
+    return hipErrorInvalidValue;
 }
 
 template <int KC>
-static void launch_ei_kc(const float* SC, const SimArgs& a, const FastShape& s, int form, hipStream_t st) {
+static hipError_t launch_ei_kc(const float* SC, const SimArgs& a, const FastShape& s, int form, hipStream_t st) {
     if constexpr (KC <= 32) {
-        if (form == 1) { launch_ei<KC, 4, 1>(SC, a, s, st); return; }
+        if (form == 1) { launch_ei<KC, 4, 1>(SC, a, s, st); return hipSuccess; }
     }
-    if (s.LS == 2) launch_ei_form<KC, 2>(SC, a, s, form, st);
-    else launch_ei_form<KC, 4>(SC, a, s, form, st);
+    if (s.LS == 2) return launch_ei_form<KC, 2>(SC, a, s, form, st);
+    return launch_ei_form<KC, 4>(SC, a, s, form, st);
 }
 
 FastShape ei_shape(int N, int LS, bool bc) {
@@ -1406,17 +1531,18 @@ FastShape ei_shape(int N, int LS, bool bc) {
 hipError_t launch_sim_ei(const float* SC, const SimArgs& a, int LS, int form, hipStream_t st) {
     const FastShape s = ei_shape(a.N, LS, form == 1);
     if (!s.ok) return hipErrorInvalidValue;
+    hipError_t e;
     switch (s.KC) {
-        case 8: launch_ei_kc<8>(SC, a, s, form, st); break;
-        case 16: launch_ei_kc<16>(SC, a, s, form, st); break;
-        case 28: launch_ei_kc<28>(SC, a, s, form, st); break;
-        case 32: launch_ei_kc<32>(SC, a, s, form, st); break;
-        case 52: launch_ei_kc<52>(SC, a, s, form, st); break;
-        case 64: launch_ei_kc<64>(SC, a, s, form, st); break;
-        case 100: launch_ei_kc<100>(SC, a, s, form, st); break;
-        default: launch_ei_kc<128>(SC, a, s, form, st); break;
+        case 8: e = launch_ei_kc<8>(SC, a, s, form, st); break;
+        case 16: e = launch_ei_kc<16>(SC, a, s, form, st); break;
+        case 28: e = launch_ei_kc<28>(SC, a, s, form, st); break;
+        case 32: e = launch_ei_kc<32>(SC, a, s, form, st); break;
+        case 52: e = launch_ei_kc<52>(SC, a, s, form, st); break;
+        case 64: e = launch_ei_kc<64>(SC, a, s, form, st); break;
+        case 100: e = launch_ei_kc<100>(SC, a, s, form, st); break;
+        default: e = launch_ei_kc<128>(SC, a, s, form, st); break;
     }
-    return hipGetLastError();
+    return e != hipSuccess ? e : hipGetLastError();
 }
 
 // FP64 E/I workgroup bound: the kernel keeps its 28 SC doubles per lane and

@@ -215,7 +215,7 @@ __global__ __launch_bounds__(512) void k_sim_mfma(const float* __restrict__ SC, 
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
                     if (node_ok[r]) {
-                        bold[((size_t)sim * a.output_ts + vol) * N + j0 + r] = bw[r].bold();
+                        bold[((size_t)sim * a.output_ts + vol) * N + j0 + r] = bw[r].bold(a.bw_k1, a.bw_k2, a.bw_k3);
                         if (vol >= a.n_vols_remove) {
                             ext[r][0] += SE[r]; ext[r][1] += IE[r]; ext[r][2] += rE[r];
                             ext[r][3] += SI[r]; ext[r][4] += II[r]; ext[r][5] += rI[r];

@@ -97,6 +97,11 @@ typedef struct bnm_config {
     int save_hemis;            /* also compute left / right FCD over the first /
                                   last (N/2)(N/2-1)/2 edges of each window FC
                                   (bnm.cu:453-495, 519-550)                       */
+    int bw_friston2003;        /* BOLD weights k1 = 7 rho, k2 = 2, k3 = 2 rho - 0.2
+                                  (the reference's BW_FRISTON2003 build,
+                                  constants.h:82-85) instead of the 3T values
+                                  3.72, 0.527, 0.53 (:86-89); env
+                                  BNM_BW_FRISTON2003                              */
 } bnm_config;
 
 /* Fills `cfg` with the reference defaults (constants.h:93-116), honouring the

@@ -70,6 +70,16 @@ def test_default_config_follows_reference_env(monkeypatch):
     assert bnm.finalize(c).n_vols_remove == 10
 
 
+def test_defaults_follow_the_reference_build():
+    """Double precision (the reference builds u_real = double unless
+    USE_FLOATS, constants.h:8-22) and a noise array covering the whole run
+    (noise_period_ms 0 -> time_steps + 1 at finalize)."""
+    c = bnm.default_config(nodes=10, n_sims=1, time_steps=60_000, bold_tr=1000, window_size=10, window_step=2)
+    assert c.precision == bnm.PREC_F64 and c.fc_precision == bnm.FC_F64
+    assert c.noise_period_ms == 0 and c.bw_friston2003 == 0
+    assert bnm.finalize(c).noise_period_ms == 60_001
+
+
 @pytest.mark.parametrize("bad", [
     dict(nodes=99),                  # exc_interhemispheric needs even N
     dict(nodes=1),
@@ -79,7 +89,7 @@ def test_default_config_follows_reference_env(monkeypatch):
     dict(nodes=1026),                # above the 1024-node limit
     dict(bold_tr=0),
     dict(window_step=0),
-    dict(noise_period_ms=0),
+    dict(noise_period_ms=-1),
     dict(i_sampling_start=9000, i_sampling_end=1000),
 ])
 def test_finalize_rejects_bad_configs(bad):

@@ -250,3 +250,26 @@ def test_bold_response_to_saturated_drive(oracle_mod):
     nu = f ** ALPHA
     q = nu * (1 - (1 - RHO) ** (1 / f)) / RHO
     assert abs(r["bold"][T, 0] - _bold((0, f, nu, q))) < 1e-6
+
+
+def test_bold_weights_friston2003(oracle_mod):
+    """The BW_FRISTON2003 weights (constants.h:82-85: k1 = 7 rho, k2 = 2,
+    k3 = 2 rho - 0.2) at the unit-step steady state, against the closed form;
+    the switch is undone afterwards."""
+    T = 30_000
+    f = 1 + GAMMA_BW * 1.0
+    nu = f ** ALPHA
+    q = nu * (1 - (1 - RHO) ** (1 / f)) / RHO
+    oracle_mod.set_bw_friston2003(True)
+    try:
+        r = oracle_mod.simulate(np.zeros((1, 1)), 0.0, [0.21], [0.15], [1.0], _const_noise(1e6, 0.0), 1,
+                                time_steps=T, bold_tr=1)
+    finally:
+        oracle_mod.set_bw_friston2003(False)
+    k1, k2, k3 = 7 * RHO, 2.0, 2 * RHO - 0.2
+    want = 100 * V0 * (k1 * (1 - q) + k2 * (1 - q / nu) + k3 * (1 - nu))
+    assert abs(r["bold"][T, 0] - want) < 1e-6
+    assert abs(r["bold"][T, 0] - _bold((0, f, nu, q))) > 1e-2   # differs from the 3T weights
+    r3 = oracle_mod.simulate(np.zeros((1, 1)), 0.0, [0.21], [0.15], [1.0], _const_noise(1e6, 0.0), 1,
+                             time_steps=T, bold_tr=1)
+    assert abs(r3["bold"][T, 0] - _bold((0, f, nu, q))) < 1e-6
