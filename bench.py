@@ -40,13 +40,13 @@ PEAK_HBM_GBS = 8000.0
 
 # HBM bytes per launch of the timed integration kernel on this workload,
 # from two rocprofv3 --pmc passes over `bench.py --steps 1 --warmup 0
-# --no-fp64 --no-cpu` (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each
-# scaled by what the counter reads for a 1 GiB device copy (FETCH_SIZE reads
-# 0.5x, WRITE_SIZE 1.0x): 79,063,470 KiB x 2 fetched + 26,524 KiB written.
-# Valid for the split_ei kernel family at 128 x 450 s; other shapes report
-# null.
+# --no-fp64 --no-cpu --no-extras` (FETCH_SIZE, then WRITE_SIZE, KiB per
+# dispatch), each scaled by what the counter reads for a 1 GiB device copy
+# (FETCH_SIZE reads 0.5x, WRITE_SIZE 1.0x): 79,399,407.5 KiB x 2 fetched +
+# 26,524.4 KiB written by the strict E/I kernel.  Valid for the split_ei
+# kernel family at 128 x 450 s; other shapes report null.
 MEASURED_TRAFFIC = {"kernel_prefix": "split_ei_f32", "n_sims": 128, "duration_ms": 450_000,
-                    "bytes": 79_063_470 * 1024 * 2 + 26_524 * 1024}
+                    "bytes": int(79_399_407.5 * 1024 * 2 + 26_524.4 * 1024)}
 
 
 def log(*a):
@@ -109,6 +109,29 @@ def run_ctx(cfg, SC, noise, emp, params, steps=1, warmup=0):
     name = c.kernel_name()
     c.close()
     return wall, integ, out, name
+
+
+def host_cores():
+    """Host cores this process may use: its CPU affinity, capped by the
+    cgroup CPU quota when one is set (a container's share of a larger host)."""
+    n = len(os.sched_getaffinity(0))
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()
+        if quota != "max":
+            n = min(n, max(1, int(-(-int(quota) // int(period)))))
+    except (OSError, ValueError):
+        pass
+    return n
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(SC, params, noise, N, T_sample, n_sims, threads):
@@ -240,8 +263,10 @@ def main():
     ap.add_argument("--duration-ms", type=int, default=450_000)
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 reference runs")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--cpu-sample-ms", type=int, default=60_000)
-    ap.add_argument("--cpu-sims", type=int, default=16)
+    ap.add_argument("--cpu-sample-ms", type=int, default=12_000,
+                    help="CPU baseline: simulated ms per simulation (past the 10 s FIC window)")
+    ap.add_argument("--cpu-sims", type=int, default=0, help="CPU baseline: simulations (0: the whole population)")
+    ap.add_argument("--no-extras", action="store_true", help="skip the uneven-batch and fcd400 keys")
     ap.add_argument("--sims", type=int, default=128, help="population size (profiling variants)")
     ap.add_argument("--precision", default="f32", choices=["f32", "bf16", "bf16x3"],
                     help="coupling precision of the timed path (profiling variants)")
@@ -318,7 +343,9 @@ def main():
     mt = MEASURED_TRAFFIC
     traffic = (mt["bytes"] if kname.startswith(mt["kernel_prefix"]) and B == mt["n_sims"] and
                T == mt["duration_ms"] else None)
-    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+    # the coupling dot runs as FP32 VALU FMAs (no MFMA in this kernel); MI355X
+    # peaks at 157.3 TFLOP/s FP32 on the vector and the matrix units alike
+    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 5), "traffic": traffic,
                 "kernel": kname, "kernel_ms": round(integ_ms, 2),
                 "hbm_gb_s": round(traffic / (integ_ms * 1e-3) / 1e9, 1) if traffic else None}
@@ -366,16 +393,36 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        cw, integ = cpu_baseline(SC, params, noise, N, args.cpu_sample_ms, args.cpu_sims, threads)
-        cpu = {"value": round(integ / cw, 2), "unit": "simulated s / wall s (incl. FIC restarts)",
+        threads = host_cores()
+        n_cpu = min(args.cpu_sims, B) if args.cpu_sims > 0 else B
+        cw, integ = cpu_baseline(SC, params, noise, N, args.cpu_sample_ms, n_cpu, threads)
+        rate = integ / cw
+        # the whole step integrates `integrated_per_step` simulated s (FIC
+        # restarts included) for `nominal` counted ones: the CPU's nominal
+        # rate at the same work mix
+        cpu = {"value": round(rate * nominal / integrated_per_step, 2), "unit": "simulated s / wall s",
                "cores": threads, "kind": "port",
-               "sample": f"oracle (FP64 C++ restatement of bnm.cpp, OpenMP, one sim per thread) on "
-                         f"the first {args.cpu_sims} simulations of pop128 for "
-                         f"{args.cpu_sample_ms / 1000:.0f} s each, integration only "
-                         f"({integ:.0f} simulated s in {cw:.1f} s)"}
+               "sample": f"oracle (FP64 C++ restatement of bnm.cpp, OpenMP, one sim per thread, "
+                         f"{threads} threads on {threads} cores of a {os.cpu_count()}-thread host, {cpu_model()}) "
+                         f"on {n_cpu} simulations of pop{B} for {args.cpu_sample_ms / 1000:g} s each with FIC "
+                         f"(integration only): {integ:.0f} simulated s (FIC restarts included) in {cw:.1f} s = "
+                         f"{rate:.1f} integrated sim-s/s; value = that rate x {nominal:.0f} nominal / "
+                         f"{integrated_per_step:.0f} integrated s of a full step (linear in T)"}
         extra["gpu_integrated_sim_s_per_s"] = round(integrated_per_step * args.steps / wall, 1)
         log(f"[bench] CPU baseline: {cpu}")
+
+    # ---- the uneven-batch and large-N FC/FCD workloads, once each -----------
+    if not args.no_extras:
+        from bnm_cuda_amd import experiments as X
+        t0 = time.perf_counter()
+        u = X.uneven_batch(fp64=not args.no_fp64, bf16x3=False, compare=False)
+        extra["uneven"] = {k: u[k] for k in ("value", "unit", "config", "graph_wall_s", "graph_device_ms",
+                                             "graph_plan", "error_vs_fp64", "bounds") if k in u}
+        log(f"[bench] uneven: {u['value']} sim-s/s in {time.perf_counter() - t0:.0f} s")
+        t0 = time.perf_counter()
+        f = X.fcd_stress(variants=("f64", "f32", "bf16x3"))
+        extra["fcd400"] = {k: f[k] for k in ("value", "unit", "config", "flop_per_launch", "gram", "bounds")}
+        log(f"[bench] fcd400: {f['value']} ms in {time.perf_counter() - t0:.0f} s")
 
     line = {
         "metric": "simulated sec/s (N=100, 128 sims, FP32 vs FP64) + FC/FCD-KS error vs FP64",

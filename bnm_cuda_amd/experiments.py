@@ -44,6 +44,13 @@ def _floor_shift(p, floor=0.001):
     return np.where(m < floor, p - (m - floor), p)
 
 
+def ks_2samp(x, y):
+    """Two-sample KS statistic sup |F_x - F_y| (numpy)."""
+    x, y = np.sort(x), np.sort(y)
+    z = np.concatenate([x, y])
+    return float(np.max(np.abs(np.searchsorted(x, z, "right") / x.size - np.searchsorted(y, z, "right") / y.size)))
+
+
 def _pearson_rows(a, emp):
     a = a - a.mean(1, keepdims=True)
     e = emp - emp.mean()
@@ -54,7 +61,7 @@ UNEVEN = [  # (N, SC seed, density, duration ms)
     (68, 10, 0.20, 900_000), (100, 11, 0.27, 700_000), (200, 12, 0.33, 500_000), (400, 13, 0.40, 300_000)]
 
 
-def uneven_batch(B=128, durations_scale=1.0, fp64=True, bf16x3=True):
+def uneven_batch(B=128, durations_scale=1.0, fp64=True, bf16x3=True, compare=True):
     pops = []
     for i, (N, seed, dens, T) in enumerate(UNEVEN):
         T = int(T * durations_scale) // 1000 * 1000
@@ -96,6 +103,20 @@ def uneven_batch(B=128, durations_scale=1.0, fp64=True, bf16x3=True):
     kernels = [c.kernel_name() for c in ctxs]
     _log(f"[uneven] graph: {wall_graph:.2f} s wall, {dev_ms:.0f} ms device, kernels {kernels}")
     m.close()
+    res = {
+        "metric": "simulated s / wall s, uneven populations in one graph launch",
+        "value": round(nominal / wall_graph, 1), "unit": "simulated s / wall s", "higher_is_better": True,
+        "n_gpus": 1, "dtype": "f32", "data": "synthetic", "vs_baseline": None,
+        "config": {"workload": "4 populations x %d sims: N/duration %s, regional wEE/wEI = base + 0.1 z, "
+                               "FIC on, FCD window 30 step 5" % (B, [(p["N"], p["T"] // 1000) for p in pops]),
+                   "nominal_sim_s": nominal, "kernels": kernels,
+                   "fic_trials": [int(o["fic_trials"].sum()) for o in out32]},
+        "graph_wall_s": round(wall_graph, 3), "graph_device_ms": round(dev_ms, 1), "graph_plan": plan,
+    }
+    if not compare:
+        for c in ctxs:
+            c.close()
+        return _uneven_errors(res, pops, out32, ctx, fp64, bf16x3)
     # the same graph with every population in its own concurrent branch
     os.environ["BNM_MULTI_BRANCHES"] = str(len(ctxs))
     try:
@@ -116,21 +137,19 @@ def uneven_batch(B=128, durations_scale=1.0, fp64=True, bf16x3=True):
         seq_each.append(round(time.perf_counter() - t1, 3))
     wall_seq = time.perf_counter() - t0
     _log(f"[uneven] sequential: {wall_seq:.2f} s ({seq_each})")
-    trials = [int(o["fic_trials"].sum()) for o in out32]
     for c in ctxs:
         c.close()
-    res = {
-        "metric": "simulated s / wall s, uneven populations in one graph launch",
-        "value": round(nominal / wall_graph, 1), "unit": "simulated s / wall s", "higher_is_better": True,
-        "n_gpus": 1, "dtype": "f32", "data": "synthetic", "vs_baseline": None,
-        "config": {"workload": "4 populations x %d sims: N/duration %s, regional wEE/wEI = base + 0.1 z, "
-                               "FIC on, FCD window 30 step 5" % (B, [(p["N"], p["T"] // 1000) for p in pops]),
-                   "nominal_sim_s": nominal, "kernels": kernels, "fic_trials": trials},
-        "graph_wall_s": round(wall_graph, 3), "graph_device_ms": round(dev_ms, 1), "graph_plan": plan,
+    res.update({
         "graph_all_concurrent_wall_s": round(wall_all, 3),
         "sequential_wall_s": round(wall_seq, 3), "sequential_each_s": seq_each,
         "graph_vs_sequential": round(wall_seq / wall_graph, 2),
-    }
+    })
+    return _uneven_errors(res, pops, out32, ctx, fp64, bf16x3)
+
+
+def _uneven_errors(res, pops, out32, ctx, fp64, bf16x3):
+    """Errors of the FP32 graph outputs against FP64 runs of the same
+    populations (and of a BF16x3-coupling run for N <= 128)."""
     if fp64:
         errs = []
         for p, o in zip(pops, out32):
@@ -140,10 +159,14 @@ def uneven_batch(B=128, durations_scale=1.0, fp64=True, bf16x3=True):
                 r = c.outputs(bold=False)
                 k64 = c.kernel_name()
             same = o["fic_trials"] == r["fic_trials"]
+            emp_fcd = p["emp"][1]
+            d_ks = max(abs(ks_2samp(emp_fcd, o["fcd"][b]) - ks_2samp(emp_fcd, r["fcd"][b]))
+                       for b in range(o["fcd"].shape[0]))
             errs.append({"N": p["N"], "f64_kernel": k64, "f64_wall_s": round(time.perf_counter() - t0, 2),
                          "same_fic_trials": int(same.sum()),
                          "mean_rE_max_abs_hz": float(np.abs(o["ext"][:, 2] - r["ext"][:, 2]).max()),
                          "fc_max_abs": float(np.abs(o["fc"] - r["fc"]).max()),
+                         "fcd_ks_max_abs": float(d_ks),
                          "cost_max_abs": float(np.abs(o["corr"] - r["corr"]).max())})
             _log(f"[uneven] FP64 N={p['N']}: {errs[-1]}")
             if bf16x3 and p["N"] <= 128:
@@ -180,7 +203,8 @@ def population_scaling(T=120_000, N=100,
     for B, p, k in runs:
         g, ee, ei = W.population(B, 3)
         cfg = _lib.default_config(nodes=N, n_sims=B, time_steps=T, bold_tr=1000, window_size=30, window_step=5,
-                                  noise_period_ms=T + 1, precision=prec[p], kernel=kern[k])
+                                  noise_period_ms=T + 1, precision=prec[p], kernel=kern[k],
+                                  fc_precision=_lib.FC_F32)
         with _lib.Context(cfg) as c:
             c.set_sc(SC)
             c.set_noise(noise)
@@ -204,7 +228,7 @@ def population_scaling(T=120_000, N=100,
             "rows": rows}
 
 
-def fcd_stress(B=16, N=400, T=864_000, TR=720, ws=44, step=1):
+def fcd_stress(B=16, N=400, T=864_000, TR=720, ws=44, step=1, variants=None):
     SC = W.synthetic_sc(N, 0.3, 0)
     g, ee, ei = W.population(B, 3)
     _log(f"[fcd400] noise {(T + 1) * 20 * N / 1e6:.0f} M draws")
@@ -234,10 +258,12 @@ def fcd_stress(B=16, N=400, T=864_000, TR=720, ws=44, step=1):
     flop_fc = 1.0 * N * (N - 1) * (ref["bold"].shape[1] + (ws // 2 * 2 + 1) * nw) * B
     flop_fcd = 1.0 * nw * (nw - 1) * np_ * B
     rows = {}
-    variants = [("f64", _lib.FC_F64, False), ("f32", _lib.FC_F32, False), ("bf16", _lib.FC_BF16, False),
-                ("bf16x3", _lib.FC_BF16X3, False), ("f32_direct", _lib.FC_F32, True),
-                ("bf16x3_direct", _lib.FC_BF16X3, True), ("f32_tile128", _lib.FC_F32, False)]
-    for name, fcp, direct in variants:
+    all_variants = [("f64", _lib.FC_F64, False), ("f32", _lib.FC_F32, False), ("bf16", _lib.FC_BF16, False),
+                    ("bf16x3", _lib.FC_BF16X3, False), ("f32_direct", _lib.FC_F32, True),
+                    ("bf16x3_direct", _lib.FC_BF16X3, True), ("f32_tile128", _lib.FC_F32, False)]
+    for name, fcp, direct in all_variants:
+        if variants is not None and name not in variants:
+            continue
         # direct: the FCD Gram z-scores inside its tile loader; tile128: the FP32
         # Gram over 128x128 output tiles (both read at context creation)
         os.environ["BNM_FCD_DIRECT"] = "1" if direct else "0"

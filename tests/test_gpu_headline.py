@@ -4,7 +4,8 @@ SC seed 0, full-length noise seed 1), at 60 s with analytical + numerical FIC,
 against the CPU oracle on identical inputs.
 
 Bars (per simulation, over the first 60 s):
-* FP32 default (strict short-chain E/I kernel, LS = 4, KC = 28): per-TR BOLD
+* FP32 default (strict short-chain E/I kernel, LS = 2 / KC = 52 and LS = 4 /
+  KC = 28, the two shapes the autotuner picks between): per-TR BOLD
   within 1e-6 relative (max |gpu - oracle| / max |oracle|), identical FIC
   trial counts, adjusted wIE equal to the oracle's, mean r_E within 0.05 Hz,
   FC within 1e-3, FCD-KS within 0.01;
@@ -61,12 +62,16 @@ def _per_sim_bold_rel(g, o):
     return np.array([np.abs(g[b] - o[b]).max() / np.abs(o[b]).max() for b in range(g.shape[0])])
 
 
-def test_headline_f32_kernel_matches_oracle(oracle_mod, case, ref, monkeypatch):
-    print("case: headline FP32 N=100 B=32 60 s", flush=True)
-    monkeypatch.setenv("BNM_EI_LS", "4")   # the tuner's pick on pop128
+@pytest.mark.parametrize("ls,shape", [(2, "LS=2,NW=4,KC=52,SS=52,threads=256"),
+                                      (4, "LS=4,NW=7,KC=28,SS=28,threads=448")])
+def test_headline_f32_kernel_matches_oracle(oracle_mod, case, ref, monkeypatch, ls, shape):
+    """Both shapes the autotuner chooses between on pop128 (within a few %
+    of each other)."""
+    print("case: headline FP32 N=100 B=32 60 s LS", ls, flush=True)
+    monkeypatch.setenv("BNM_EI_LS", str(ls))
     name, g = _run(case, bnm.PREC_F32)
     print("kernel", name)
-    assert name == "split_ei_f32(LS=4,NW=7,KC=28,SS=28,threads=448,short1,strict)"
+    assert name == f"split_ei_f32({shape},short1,strict)"
     rel = _per_sim_bold_rel(g["bold"], ref["bold"])
     d_re = np.abs(g["ext"][:, 2] - ref["ext"][:, 2]).max()
     d_fc = np.abs(g["fc"] - ref["fc"]).max()
