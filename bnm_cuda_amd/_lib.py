@@ -28,7 +28,7 @@ FC_F64, FC_F32, FC_BF16, FC_BF16X3 = 0, 1, 2, 3
 # every symbol include/bnm_hip.h declares
 EXPORTS = (
     "bnm_default_config", "bnm_finalize_config", "bnm_output_ts", "bnm_n_pairs", "bnm_n_windows",
-    "bnm_window_bounds", "bnm_regional_params", "bnm_analytical_fic", "bnm_make_noise", "bnm_make_noise_chunked", "bnm_run_simulations", "bnm_ctx_create",
+    "bnm_window_bounds", "bnm_regional_params", "bnm_analytical_fic", "bnm_make_noise", "bnm_make_noise_chunked", "bnm_run_simulations", "bnm_run_simulations_noalloc", "bnm_ctx_create",
     "bnm_ctx_destroy", "bnm_ctx_set_sc", "bnm_ctx_set_noise", "bnm_ctx_set_empirical",
     "bnm_ctx_set_params", "bnm_ctx_run", "bnm_ctx_set_bold", "bnm_ctx_analyze", "bnm_ctx_get_hemi_fcd", "bnm_ctx_get_outputs", "bnm_ctx_timings",
     "bnm_ctx_kernel_name", "bnm_ctx_tune_log", "bnm_multi_create", "bnm_multi_run",
@@ -90,6 +90,7 @@ def lib():
     L.bnm_make_noise_chunked.argtypes = [C.c_uint, C.c_longlong, pf, C.c_int]
     L.bnm_run_simulations.argtypes = [pc, pd, pd, pd, pd, pd, pd, pd, pd, pf, C.c_longlong,
                                       pd, C.c_int, pd, C.c_int] + [pd] * 7 + [pi, pi, pd, pd]
+    L.bnm_run_simulations_noalloc.argtypes = L.bnm_run_simulations.argtypes
     L.bnm_ctx_create.argtypes = [pc, pi]
     L.bnm_ctx_create.restype = vp
     L.bnm_ctx_destroy.argtypes = [vp]
@@ -516,8 +517,10 @@ class Multi:
 
 
 def run_simulations(cfg: Config, SC, G, wEE, wEI, wIE=None, wIE_fic=None, *, noise,
-                    emp_fc=None, emp_fcd=None, bold=True):
-    """One-shot call of bnm_run_simulations (the reference's run_simulations)."""
+                    emp_fc=None, emp_fcd=None, bold=True, noalloc=False):
+    """One-shot call of bnm_run_simulations (the reference's run_simulations),
+    or with noalloc of bnm_run_simulations_noalloc (device buffers from the
+    library's static scratch array, no device allocation)."""
     cfg = finalize(Config.from_buffer_copy(cfg))
     B, N = cfg.n_sims, cfg.nodes
     ots = output_ts(cfg.time_steps, cfg.bold_tr)
@@ -541,7 +544,8 @@ def run_simulations(cfg: Config, SC, G, wEE, wEI, wIE=None, wIE_fic=None, *, noi
     trials = np.zeros(B, np.int32)
     fc = np.zeros((B, max(npairs, 1)))
     fcd = np.zeros((B, max(nwp, 1)))
-    _chk(lib().bnm_run_simulations(
+    fn = lib().bnm_run_simulations_noalloc if noalloc else lib().bnm_run_simulations
+    _chk(fn(
         C.byref(cfg), _p(corr, C.c_double), _p(pen, C.c_double), _p(G, C.c_double),
         _p(wEE, C.c_double), _p(wEI, C.c_double), _p(wIE, C.c_double), _p(wf, C.c_double),
         _p(SC, C.c_double), _p(noise, C.c_float), noise.size, _p(emp_fc, C.c_double),

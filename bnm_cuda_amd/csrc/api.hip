@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <string>
 #include <thread>
@@ -70,6 +71,12 @@ struct Variant {
     int ls, ks, mode;
 };
 
+// Scratch of the no-allocation entry point: one static device array per GPU
+// (each device the library runs on holds its own copy until the process
+// ends).  Host code never uses the array's name as an address: the address is
+// taken once per device with hipGetSymbolAddress (scratch_base below).
+static __device__ __align__(256) unsigned char scratch[BNM_NOALLOC_SCRATCH_BYTES];
+
 struct bnm_ctx {
     bnm_config cfg{};
     int N = 0, B = 0, ots = 0, np = 0, nw = 0, nseg = 0, m = 0;
@@ -126,7 +133,40 @@ struct bnm_ctx {
     hipGraphExec_t graph_exec = nullptr;
     bool have_sc = false, have_noise = false, have_emp = false, have_params = false;
     int fast_ls = 0, fast_ks = 0, stream_ks = 0, force_stream = 0, no_ei = 0, ei_ls = 0, ei_bc = 0, ei_form = -1, no_hyb = 0, hyb_ks = 0, hyb_bc = -1, hyb_rr = -1;
+    // no-allocation mode (bnm_run_simulations_noalloc): every device buffer is
+    // carved out of `arena` (the library's static scratch array), the null
+    // stream is used and no events, tuner windows or graphs are created
+    unsigned char* arena = nullptr;
+    size_t arena_cap = 0, arena_used = 0;
 };
+
+// Device buffer of `bytes` for the context: from the arena (256-byte aligned
+// bump allocation, BNM_E_SIZE when it does not fit) or hipMalloc.
+static int ctx_alloc_bytes(bnm_ctx* x, void** p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) bytes = 1;
+    if (x->arena) {
+        const size_t off = (x->arena_used + 255) & ~(size_t)255;
+        if (off > x->arena_cap || bytes > x->arena_cap - off)
+            return fail(BNM_E_SIZE, "configuration needs more than the " + std::to_string(x->arena_cap >> 20) +
+                                        " MiB no-allocation scratch (" + std::to_string((off + bytes) >> 20) +
+                                        "+ MiB so far)");
+        *p = x->arena + off;
+        x->arena_used = off + bytes;
+        return BNM_OK;
+    }
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return fail(BNM_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return BNM_OK;
+}
+
+template <class T>
+static int cdalloc(bnm_ctx* x, T** p, size_t n) {
+    void* v = nullptr;
+    const int r = ctx_alloc_bytes(x, &v, (n ? n : 1) * sizeof(T));
+    *p = static_cast<T*>(v);
+    return r;
+}
 
 extern "C" {
 
@@ -285,8 +325,9 @@ void bnm_ctx_destroy(bnm_ctx* x) {
                     x->seg_start, x->seg_len, x->pidx, x->mean, x->inv, x->wmean, x->winv, x->fc,
                     x->wfc, x->wz, x->wzlo, x->fcd, x->emp_c, x->ks_u, x->ks_fe, x->bins, x->nanf, x->corr, x->pen,
                     x->fcd_h[0], x->fcd_h[1], x->wmean_h[0], x->wmean_h[1], x->winv_h[0], x->winv_h[1]};
-    for (void* p : bufs)
-        if (p) (void)hipFree(p);
+    if (!x->arena)
+        for (void* p : bufs)
+            if (p) (void)hipFree(p);
     for (auto& e : x->ev)
         if (e) (void)hipEventDestroy(e);
     if (x->graph_exec) (void)hipGraphExecDestroy(x->graph_exec);
@@ -299,33 +340,33 @@ static int ctx_alloc(bnm_ctx* x) {
     const size_t N = x->N, B = x->B;
     const size_t rsz = x->f64 ? 8 : 4, fsz = x->fc64 ? 8 : 4;
     int r;
-    if ((r = dalloc(&x->SC64, N * N))) return r;
-    if ((r = dalloc(&x->SC32, N * N))) return r;
-    if ((r = dalloc(&x->SCt64, N * N))) return r;
+    if ((r = cdalloc(x, &x->SC64, N * N))) return r;
+    if ((r = cdalloc(x, &x->SC32, N * N))) return r;
+    if ((r = cdalloc(x, &x->SCt64, N * N))) return r;
     // kSCtPad zero rows after the N rows: the hybrid kernel's padded k-ranges
     // read them without bounds checks
-    if ((r = dalloc(&x->SCt32, (N + kSCtPad) * N))) return r;
+    if ((r = cdalloc(x, &x->SCt32, (N + kSCtPad) * N))) return r;
     HIPCHK(hipMemset(x->SCt32, 0, (N + kSCtPad) * N * sizeof(float)));
-    if ((r = dalloc(&x->SCc32, (N + kSCtPad) * N))) return r;
+    if ((r = cdalloc(x, &x->SCc32, (N + kSCtPad) * N))) return r;
     HIPCHK(hipMemset(x->SCc32, 0, (N + kSCtPad) * N * sizeof(float)));
     if (x->f64) {
-        if ((r = dalloc(&x->SCc64, (N + kSCtPad) * N))) return r;
+        if ((r = cdalloc(x, &x->SCc64, (N + kSCtPad) * N))) return r;
         HIPCHK(hipMemset(x->SCc64, 0, (N + kSCtPad) * N * sizeof(double)));
     }
-    if ((r = dalloc(&x->G, B))) return r;
-    if ((r = dalloc(&x->wEE, B * N))) return r;
-    if ((r = dalloc(&x->wEI, B * N))) return r;
-    if ((r = dalloc(&x->wIE, B * N))) return r;
-    if ((r = dalloc(&x->wIEfic, B * N))) return r;
-    if ((r = dalloc(&x->wIEadj, B * N))) return r;
-    if ((r = dalloc(&x->wIEscratch, B * N))) return r;
-    HIPCHK(hipMalloc(&x->bold, B * x->ots * N * rsz));
-    if ((r = dalloc(&x->ext, B * 6 * N))) return r;
-    if ((r = dalloc(&x->failed, B))) return r;
-    if ((r = dalloc(&x->trials, B))) return r;
-    if ((r = dalloc(&x->unstable, B))) return r;
-    if ((r = dalloc(&x->corr, B))) return r;
-    if ((r = dalloc(&x->pen, B))) return r;
+    if ((r = cdalloc(x, &x->G, B))) return r;
+    if ((r = cdalloc(x, &x->wEE, B * N))) return r;
+    if ((r = cdalloc(x, &x->wEI, B * N))) return r;
+    if ((r = cdalloc(x, &x->wIE, B * N))) return r;
+    if ((r = cdalloc(x, &x->wIEfic, B * N))) return r;
+    if ((r = cdalloc(x, &x->wIEadj, B * N))) return r;
+    if ((r = cdalloc(x, &x->wIEscratch, B * N))) return r;
+    if ((r = ctx_alloc_bytes(x, &x->bold, B * x->ots * N * rsz))) return r;
+    if ((r = cdalloc(x, &x->ext, B * 6 * N))) return r;
+    if ((r = cdalloc(x, &x->failed, B))) return r;
+    if ((r = cdalloc(x, &x->trials, B))) return r;
+    if ((r = cdalloc(x, &x->unstable, B))) return r;
+    if ((r = cdalloc(x, &x->corr, B))) return r;
+    if ((r = cdalloc(x, &x->pen, B))) return r;
     if (!c.sim_only) {
         std::vector<int> st(x->nseg), ln(x->nseg);
         st[0] = c.n_vols_remove;
@@ -336,8 +377,8 @@ static int ctx_alloc(bnm_ctx* x) {
             st[w + 1] = ws[w];
             ln[w + 1] = we[w] - ws[w] + 1;
         }
-        if ((r = dalloc(&x->seg_start, x->nseg))) return r;
-        if ((r = dalloc(&x->seg_len, x->nseg))) return r;
+        if ((r = cdalloc(x, &x->seg_start, x->nseg))) return r;
+        if ((r = cdalloc(x, &x->seg_len, x->nseg))) return r;
         HIPCHK(hipMemcpy(x->seg_start, st.data(), x->nseg * sizeof(int), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(x->seg_len, ln.data(), x->nseg * sizeof(int), hipMemcpyHostToDevice));
         // pair table (run_CMAES.cpp:494-524): pidx[i*N+j] = position of pair (i,j), i > j
@@ -349,34 +390,35 @@ static int ctx_alloc(bnm_ctx* x) {
                 if (c.exc_interhemispheric && ((i < rh) ^ (j < rh))) continue;
                 pidx[(size_t)i * N + j] = p++;
             }
-        if ((r = dalloc(&x->pidx, N * N))) return r;
+        if ((r = cdalloc(x, &x->pidx, N * N))) return r;
         HIPCHK(hipMemcpy(x->pidx, pidx.data(), N * N * sizeof(int), hipMemcpyHostToDevice));
-        if ((r = dalloc(&x->mean, B * x->nseg * N))) return r;
-        if ((r = dalloc(&x->inv, B * x->nseg * N))) return r;
-        HIPCHK(hipMalloc(&x->fc, B * x->np * fsz));
-        if ((r = dalloc(&x->emp_c, x->np))) return r;
+        if ((r = cdalloc(x, &x->mean, B * x->nseg * N))) return r;
+        if ((r = cdalloc(x, &x->inv, B * x->nseg * N))) return r;
+        if ((r = ctx_alloc_bytes(x, &x->fc, B * x->np * fsz))) return r;
+        if ((r = cdalloc(x, &x->emp_c, x->np))) return r;
         if (!c.no_fcd) {
-            HIPCHK(hipMalloc(&x->wfc, B * (size_t)x->nw * x->np * fsz));
-            if ((r = dalloc(&x->wmean, B * x->nw))) return r;
-            if ((r = dalloc(&x->winv, B * x->nw))) return r;
+            if ((r = ctx_alloc_bytes(x, &x->wfc, B * (size_t)x->nw * x->np * fsz))) return r;
+            if ((r = cdalloc(x, &x->wmean, B * x->nw))) return r;
+            if ((r = cdalloc(x, &x->winv, B * x->nw))) return r;
             if (!x->fc64 && !x->fcd_direct) {
                 const size_t zn = B * (size_t)x->nw * ((x->np + 63) / 64 * 64);
-                HIPCHK(hipMalloc(&x->wz, zn * (x->fcbf16 ? 2 : 4)));
-                if (x->fcsplit) HIPCHK(hipMalloc(&x->wzlo, zn * 2));
+                if ((r = ctx_alloc_bytes(x, &x->wz, zn * (x->fcbf16 ? 2 : 4)))) return r;
+                if (x->fcsplit && (r = cdalloc(x, &x->wzlo, zn))) return r;
             }
-            HIPCHK(hipMalloc(&x->fcd, B * (size_t)x->nwp * fsz));
+            if ((r = ctx_alloc_bytes(x, &x->fcd, B * (size_t)x->nwp * fsz))) return r;
             if (c.save_hemis) {
                 x->n_hemi = (x->N / 2) * (x->N / 2 - 1) / 2;
                 for (int h = 0; h < 2; ++h) {
-                    HIPCHK(hipMalloc(&x->fcd_h[h], B * (size_t)x->nwp * fsz));
-                    if ((r = dalloc(&x->wmean_h[h], B * x->nw))) return r;
-                    if ((r = dalloc(&x->winv_h[h], B * x->nw))) return r;
+                    if ((r = ctx_alloc_bytes(x, &x->fcd_h[h], B * (size_t)x->nwp * fsz))) return r;
+                    if ((r = cdalloc(x, &x->wmean_h[h], B * x->nw))) return r;
+                    if ((r = cdalloc(x, &x->winv_h[h], B * x->nw))) return r;
                 }
             }
-            if ((r = dalloc(&x->nanf, B))) return r;
+            if ((r = cdalloc(x, &x->nanf, B))) return r;
         }
     }
-    for (auto& e : x->ev) HIPCHK(hipEventCreate(&e));
+    if (!x->arena)
+        for (auto& e : x->ev) HIPCHK(hipEventCreate(&e));
     return BNM_OK;
 }
 
@@ -426,7 +468,9 @@ int bnm_analytical_fic(int nodes, int n_sims, const double* SC, const double* G,
     return r;
 }
 
-bnm_ctx* bnm_ctx_create(const bnm_config* cfg, int* status) {
+// Context on the current device; with `arena` every device buffer comes from
+// that memory (no-allocation mode, see bnm_run_simulations_noalloc).
+static bnm_ctx* ctx_create(const bnm_config* cfg, int* status, unsigned char* arena, size_t arena_cap) {
     auto set = [&](int s) { if (status) *status = s; };
     if (!cfg) { set(fail(BNM_E_ARG, "null config")); return nullptr; }
     if (int r = check_device()) { set(r); return nullptr; }
@@ -501,7 +545,13 @@ bnm_ctx* bnm_ctx_create(const bnm_config* cfg, int* status) {
     }
     x->autotune = c.autotune;
     x->tune_ms = env_int("BNM_TUNE_MS", 300);
-    if (hipStreamCreateWithFlags(&x->st, hipStreamNonBlocking) != hipSuccess) {
+    if (arena) {
+        // the first candidate kernel, no timing windows, no graph, null stream
+        x->arena = arena;
+        x->arena_cap = arena_cap;
+        x->autotune = 0;
+        x->cfg.use_graph = 0;
+    } else if (hipStreamCreateWithFlags(&x->st, hipStreamNonBlocking) != hipSuccess) {
         set(fail(BNM_E_HIP, "hipStreamCreate failed"));
         delete x;
         return nullptr;
@@ -515,6 +565,8 @@ bnm_ctx* bnm_ctx_create(const bnm_config* cfg, int* status) {
     set(BNM_OK);
     return x;
 }
+
+bnm_ctx* bnm_ctx_create(const bnm_config* cfg, int* status) { return ctx_create(cfg, status, nullptr, 0); }
 
 int bnm_ctx_set_sc(bnm_ctx* x, const double* SC) {
     if (!x || !SC) return fail(BNM_E_ARG, "null argument");
@@ -552,9 +604,9 @@ int bnm_ctx_set_noise(bnm_ctx* x, const float* noise, long long len) {
     const long long need = x->cfg.noise_period_ms * 10LL * x->N * 2;
     if (len < need) return fail(BNM_E_ARG, "noise array shorter than noise_period_ms*10*N*2");
     if (x->noise_cap < need) {
-        if (x->noise) (void)hipFree(x->noise);
+        if (x->noise && !x->arena) (void)hipFree(x->noise);
         x->noise = nullptr;
-        HIPCHK(hipMalloc(&x->noise, need * sizeof(float)));
+        if (int r = cdalloc(x, &x->noise, (size_t)need)) return r;
         x->noise_cap = need;
     }
     HIPCHK(hipMemcpy(x->noise, noise, need * sizeof(float), hipMemcpyHostToDevice));
@@ -590,14 +642,18 @@ int bnm_ctx_set_empirical(bnm_ctx* x, const double* emp_fc, int n_fc, const doub
             uu.push_back(u[i]);
             fe.push_back((double)(i + 1) / n_fcd);
         }
-        if (x->ks_u) { (void)hipFree(x->ks_u); x->ks_u = nullptr; }
-        if (x->ks_fe) { (void)hipFree(x->ks_fe); x->ks_fe = nullptr; }
-        if (x->bins) { (void)hipFree(x->bins); x->bins = nullptr; }
+        if (!x->arena) {
+            if (x->ks_u) (void)hipFree(x->ks_u);
+            if (x->ks_fe) (void)hipFree(x->ks_fe);
+            if (x->bins) (void)hipFree(x->bins);
+        }
+        x->ks_u = x->ks_fe = nullptr;
+        x->bins = nullptr;
         x->m = (int)uu.size();
         int r;
-        if ((r = dalloc(&x->ks_u, uu.size()))) return r;
-        if ((r = dalloc(&x->ks_fe, fe.size()))) return r;
-        if ((r = dalloc(&x->bins, (size_t)x->B * (2 * x->m + 1)))) return r;
+        if ((r = cdalloc(x, &x->ks_u, uu.size()))) return r;
+        if ((r = cdalloc(x, &x->ks_fe, fe.size()))) return r;
+        if ((r = cdalloc(x, &x->bins, (size_t)x->B * (2 * x->m + 1)))) return r;
         HIPCHK(hipMemcpy(x->ks_u, uu.data(), uu.size() * sizeof(double), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(x->ks_fe, fe.data(), fe.size() * sizeof(double), hipMemcpyHostToDevice));
     }
@@ -1019,6 +1075,14 @@ int bnm_ctx_run(bnm_ctx* x) {
         return fail(BNM_E_ARG, "SC, noise, parameters and the empirical target must be set first");
     int r = choose_variant(x);
     if (r) return r;
+    if (x->arena) {
+        r = enqueue_run(x, x->st, false);
+        if (r) return r;
+        HIPCHK(hipStreamSynchronize(x->st));
+        for (int i = 0; i < 6; ++i) x->timings[i] = -1;
+        x->have_bold = true;
+        return BNM_OK;
+    }
     if (x->cfg.use_graph) {
         if (!x->graph_exec) {
             hipGraph_t gph = nullptr;
@@ -1291,6 +1355,54 @@ int bnm_ctx_timings(bnm_ctx* x, double* ms, int n) {
 }
 
 const char* bnm_ctx_kernel_name(bnm_ctx* x) { return x ? x->kname.c_str() : ""; }
+
+namespace {
+std::mutex g_scratch_mu;             // serialises no-allocation calls in this process
+void* g_scratch_ptr[256] = {};       // scratch address per device ordinal
+
+// Device address of the current device's scratch array (taken once per
+// device, with that device current).  Called with g_scratch_mu held.
+int scratch_base(void** out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 256) return fail(BNM_E_UNSUP, "device ordinal beyond 255");
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return fail(BNM_E_HIP, "hipGetSymbolAddress returned a null scratch address");
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return BNM_OK;
+}
+}  // namespace
+
+int bnm_run_simulations_noalloc(const bnm_config* cfg, double* corr, double* fic_penalties,
+                                const double* G_list, const double* w_EE_list, const double* w_EI_list,
+                                const double* w_IE_list, double* w_IE_fic, const double* SC,
+                                const float* noise, long long noise_len, const double* emp_FC_tril,
+                                int n_emp_fc, const double* emp_FCD_tril, int n_emp_fcd, double* BOLD,
+                                double* S_E, double* I_E, double* r_E, double* S_I, double* I_I,
+                                double* r_I, int* FIC_failed, int* fic_n_trials, double* fc_trils,
+                                double* fcd_trils) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (int r = check_device()) return r;
+    void* base = nullptr;
+    if (int r = scratch_base(&base)) return r;
+    int st = BNM_OK;
+    bnm_ctx* x = ctx_create(cfg, &st, static_cast<unsigned char*>(base), BNM_NOALLOC_SCRATCH_BYTES);
+    if (!x) return st;
+    int r = bnm_ctx_set_sc(x, SC);
+    if (!r) r = bnm_ctx_set_noise(x, noise, noise_len);
+    if (!r) r = bnm_ctx_set_empirical(x, emp_FC_tril, n_emp_fc, emp_FCD_tril, n_emp_fcd);
+    if (!r) r = bnm_ctx_set_params(x, G_list, w_EE_list, w_EI_list, w_IE_list, w_IE_fic);
+    if (!r) r = bnm_ctx_run(x);
+    if (!r)
+        r = bnm_ctx_get_outputs(x, corr, fic_penalties, w_IE_fic, BOLD, S_E, I_E, r_E, S_I, I_I,
+                                r_I, FIC_failed, fic_n_trials, fc_trils, fcd_trils);
+    bnm_ctx_destroy(x);
+    return r;
+}
 
 int bnm_run_simulations(const bnm_config* cfg, double* corr, double* fic_penalties,
                         const double* G_list, const double* w_EE_list, const double* w_EI_list,

@@ -176,6 +176,35 @@ int bnm_run_simulations(const bnm_config* cfg, double* corr, double* fic_penalti
                         int* FIC_failed, int* fic_n_trials,
                         double* fc_trils, double* fcd_trils);
 
+/* No counterpart in the reference.  The same run as bnm_run_simulations for
+ * callers that cannot allocate device memory (code that prepares or runs
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): every device buffer of the run (device copies of SC, noise,
+ * parameters and the empirical target; BOLD, moments, FC, window FC, FCD,
+ * KS bins, outputs) is carved out of one static device array of
+ * BNM_NOALLOC_SCRATCH_BYTES per GPU, held from library load until the process
+ * ends; the call makes no hipMalloc / hipFree, creates no stream, event or
+ * graph, and picks the first candidate integration kernel without timing
+ * windows (set autotune = 0 in bnm_run_simulations for the same choice).  A
+ * configuration whose buffers exceed the array returns BNM_E_SIZE before any
+ * kernel runs.  Calls from the threads of one process are serialised by a
+ * lock; the array keeps the last call's contents and every call rewrites what
+ * it reads, so results do not depend on earlier calls.  Calls that overlap on
+ * one GPU from outside that lock (another copy of the library in the same
+ * process) would race on the array.  Arguments and outputs as
+ * bnm_run_simulations. */
+#define BNM_NOALLOC_SCRATCH_BYTES (256u << 20)
+int bnm_run_simulations_noalloc(const bnm_config* cfg, double* corr, double* fic_penalties,
+                                const double* G_list, const double* w_EE_list, const double* w_EI_list,
+                                const double* w_IE_list, double* w_IE_fic, const double* SC,
+                                const float* noise, long long noise_len,
+                                const double* emp_FC_tril, int n_emp_fc,
+                                const double* emp_FCD_tril, int n_emp_fcd,
+                                double* BOLD, double* S_E, double* I_E, double* r_E,
+                                double* S_I, double* I_I, double* r_I,
+                                int* FIC_failed, int* fic_n_trials,
+                                double* fc_trils, double* fcd_trils);
+
 /* ---- persistent context: device buffers kept across calls (a CMA-ES run
  * calls run_simulations once per generation with the same SC and noise) ---- */
 typedef struct bnm_ctx bnm_ctx;
