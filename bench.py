@@ -171,15 +171,44 @@ def copy_calibration():
                       "gb_s": round(8 * n / dt / 1e9, 1)}), flush=True)
 
 
-def surrogate_experiment(args):
-    """Surrogate-guided CMA-ES (learned FIC initialiser + fit pre-screening)
-    against plain CMA-ES, both through the run_cmaes program on the same
-    synthetic inputs (N=100, 450 s, one z map -> 5 free parameters, FP32):
-    full simulations until the surrogate run's best cost reaches the plain
-    run's final best, FIC restarts per simulation, and the plain re-run of
-    each run's best point."""
+def _cmaes_run(exe, work, tag, lam, T, its, args, env_extra):
+    """One run_cmaes search on the inputs in `work`; returns its progress rows,
+    wall time, the plain re-run cost of its best point and its log text."""
     import csv
     import subprocess
+    out = os.path.join(work, tag)
+    env = dict(os.environ, BNM_PRECISION="f32", BNM_NOISE_CHUNKED="1", **env_extra)
+    cmd = [exe, os.path.join(work, "SC.txt"), out, os.path.join(work, "emp_FCtril.txt"),
+           os.path.join(work, "emp_FCDtril.txt"), os.path.join(work, "maps.txt"), "100", "0.5-4",
+           "0.05-0.75", "0.05-0.75", "0", "wee-wei", str(T), "1000", str(args.step), str(args.window), "1",
+           str(lam), str(its), str(env_extra.get("_seed", args.seed_mw))]
+    env.pop("_seed", None)
+    log(f"[surrogate] {tag}: {' '.join(cmd[7:])} {env_extra}")
+    t0 = time.perf_counter()
+    r = subprocess.run(cmd, env=env, stdout=sys.stderr, stderr=sys.stderr, timeout=3000)
+    if r.returncode != 0:
+        raise RuntimeError(f"run_cmaes {tag} failed with {r.returncode}")
+    wall = time.perf_counter() - t0
+    cdir = os.path.join(out, "cmaes_multimaps_gpu")
+    prog = [f for f in os.listdir(cdir) if f.endswith("_progress.csv")][0]
+    with open(os.path.join(cdir, prog)) as f:
+        rows = list(csv.DictReader(f))
+    logf = [f for f in os.listdir(cdir) if f.endswith(".txt")][0]
+    text = open(os.path.join(cdir, logf)).read()
+    tail = text.split("Best feasible parameters: ")[1]
+    rerun = -float(tail.split("gof=")[1].split(",")[0])
+    pen = float(tail.split("fic_penalty=")[1].split(",")[0])
+    return rows, wall, rerun + pen, text
+
+
+def surrogate_experiment(args):
+    """Surrogate-guided CMA-ES (fit pre-screening of a pool of proposals,
+    learned FIC start with the plain-consistency guard) against plain CMA-ES,
+    both through the run_cmaes program on the same synthetic inputs (N=100,
+    450 s, one z map -> 5 free parameters, FP32), for each CMA-ES seed: full
+    simulations until the surrogate run's plain-judged best reaches the plain
+    run's final best, FIC restarts per full simulation, and the plain re-run
+    of each run's best point."""
     import tempfile
     from bnm_cuda_amd import inputs as I
     T, lam, its = args.duration_ms, 128, args.generations + 1
@@ -187,72 +216,93 @@ def surrogate_experiment(args):
     log(f"[surrogate] writing inputs to {work}")
     I.write_inputs(work, nodes=100, duration_ms=T, window=args.window, step=args.step, chunked_noise=True)
     exe = os.path.join(ROOT, "bnm_cuda_amd", "run_cmaes")
+    surr_env = {"BNM_SURROGATE": "1", "BNM_SURR_POOL": str(args.pool),
+                "BNM_SURR_FIC": str(args.surrogate_fic), "BNM_SURR_UNSCREENED": str(args.unscreened),
+                "BNM_SURR_GUARD": str(args.guard)}
+    if args.fic_model:
+        surr_env["BNM_SURR_FIC_MODEL"] = os.path.abspath(args.fic_model)
 
-    def run(tag, lam_, env_extra):
-        out = os.path.join(work, tag)
-        env = dict(os.environ, BNM_PRECISION="f32", BNM_NOISE_CHUNKED="1", **env_extra)
-        cmd = [exe, os.path.join(work, "SC.txt"), out, os.path.join(work, "emp_FCtril.txt"),
-               os.path.join(work, "emp_FCDtril.txt"), os.path.join(work, "maps.txt"), "100", "0.5-4",
-               "0.05-0.75", "0.05-0.75", "0", "wee-wei", str(T), "1000", str(args.step), str(args.window), "1",
-               str(lam_), str(its),
-               str(args.seed_mw)]
-        log(f"[surrogate] {tag}: {' '.join(cmd[7:])} {env_extra}")
-        t0 = time.perf_counter()
-        r = subprocess.run(cmd, env=env, stdout=sys.stderr, stderr=sys.stderr, timeout=3000)
-        if r.returncode != 0:
-            raise RuntimeError(f"run_cmaes {tag} failed with {r.returncode}")
-        wall = time.perf_counter() - t0
-        cdir = os.path.join(out, "cmaes_multimaps_gpu")
-        prog = [f for f in os.listdir(cdir) if f.endswith("_progress.csv")][0]
-        with open(os.path.join(cdir, prog)) as f:
-            rows = list(csv.DictReader(f))
-        logf = [f for f in os.listdir(cdir) if f.endswith(".txt")][0]
-        text = open(os.path.join(cdir, logf)).read()
-        tail = text.split("Best feasible parameters: ")[1]
-        rerun = -float(tail.split("gof=")[1].split(",")[0])
-        pen = float(tail.split("fic_penalty=")[1].split(",")[0])
-        return rows, wall, rerun + pen, text
+    def restarts(rows):
+        sims = [int(r["full_sims"]) for r in rows]
+        per = [float(r["mean_fic_trials"]) for r in rows]
+        n = [sims[0]] + [b - a for a, b in zip(sims, sims[1:])]
+        return sum(p * k for p, k in zip(per, n)) / sims[-1]
 
-    plain, w_plain, plain_rerun, _ = run("plain", lam, {})
-    surr, w_surr, surr_rerun, surr_text = run("surrogate", args.surrogate_lambda,
-                                              {"BNM_SURROGATE": "1", "BNM_SURR_POOL": str(args.pool),
-                                               "BNM_SURR_FIC": str(args.surrogate_fic),
-                                               "BNM_SURR_UNSCREENED": str(args.unscreened)})
-    # every cost compared here is a plain-method simulation: the plain run's
-    # own costs, and the surrogate run's plain re-runs of its generation bests
-    best_plain = float(plain[-1]["best_plain_cost"])
-    sims_plain_total = int(plain[-1]["full_sims"])
-    sims_plain_first = next(int(r["full_sims"]) for r in plain if float(r["best_plain_cost"]) <= best_plain)
-    reach = [int(r["full_sims"]) for r in surr if float(r["best_plain_cost"]) <= best_plain]
+    seeds = [int(v) for v in str(args.seeds).split(",")]
+    rows_out = []
+    for seed in seeds:
+        plain, w_plain, plain_rerun, _ = _cmaes_run(exe, work, f"plain{seed}", lam, T, its, args, {"_seed": seed})
+        surr, w_surr, surr_rerun, text = _cmaes_run(exe, work, f"surrogate{seed}", args.surrogate_lambda, T, its,
+                                                    args, dict(surr_env, _seed=seed))
+        best_plain = float(plain[-1]["best_plain_cost"])
+        first_plain = next(int(r["full_sims"]) for r in plain if float(r["best_plain_cost"]) <= best_plain)
+        reach = [int(r["full_sims"]) for r in surr if float(r["best_plain_cost"]) <= best_plain]
+        row = {"seed": seed, "plain_best": best_plain, "plain_sims_total": int(plain[-1]["full_sims"]),
+               "plain_sims_to_best": first_plain, "surrogate_sims_to_plain_best": reach[0] if reach else None,
+               "ratio": round(first_plain / reach[0], 3) if reach else None,
+               "ratio_vs_total": round(int(plain[-1]["full_sims"]) / reach[0], 3) if reach else None,
+               "surrogate_final_plain_best": float(surr[-1]["best_plain_cost"]),
+               "surrogate_sims_total": int(surr[-1]["full_sims"]),
+               "restarts_per_sim_plain": round(restarts(plain), 4),
+               "restarts_per_sim_surrogate": round(restarts(surr), 4),
+               "guard_sims": int(sum(int(r.get("guard_sims", 0) or 0) for r in surr)),
+               "generations_prescreened": int(sum(int(r["surrogate_used"]) for r in surr)),
+               "best_rerun_plain_cost": {"plain": plain_rerun, "surrogate": surr_rerun},
+               "wall_s": {"plain": round(w_plain, 1), "surrogate": round(w_surr, 1)},
+               "fic_initialiser": text.split("Surrogate FIC initialiser: ")[1].split("\n")[0]
+               if "Surrogate FIC initialiser: " in text else None}
+        if args.compare_no_fic:
+            nf, w_nf, nf_rerun, _ = _cmaes_run(exe, work, f"surrogate_nofic{seed}", args.surrogate_lambda, T, its,
+                                               args, dict(surr_env, BNM_SURR_FIC="0", _seed=seed))
+            reach_nf = [int(r["full_sims"]) for r in nf if float(r["best_plain_cost"]) <= best_plain]
+            row["no_fic_start"] = {
+                "surrogate_sims_to_plain_best": reach_nf[0] if reach_nf else None,
+                "ratio": round(first_plain / reach_nf[0], 3) if reach_nf else None,
+                "final_plain_best": float(nf[-1]["best_plain_cost"]), "sims_total": int(nf[-1]["full_sims"]),
+                "restarts_per_sim": round(restarts(nf), 4),
+                "simulated_s": float(nf[-1]["simulated_s"]), "wall_s": round(w_nf, 1)}
+            row["simulated_s"] = {"plain": float(plain[-1]["simulated_s"]), "surrogate": float(surr[-1]["simulated_s"])}
+        log(f"[surrogate] seed {seed}: {row}")
+        rows_out.append(row)
+    ratios = [r["ratio"] for r in rows_out if r["ratio"] is not None]
     line = {
         "metric": "full simulations for surrogate-guided CMA-ES to reach the plain CMA-ES best fit",
-        "value": reach[0] if reach else None, "unit": "full simulations", "higher_is_better": False,
+        "value": round(float(np.mean(ratios)), 3) if ratios else None,
+        "unit": "x fewer full simulations (mean over seeds)", "higher_is_better": True,
         "n_gpus": 1, "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"CMA-ES N=100, {T / 1000:.0f} s, 5 free parameters (G, wEE, wEI, "
                                f"weescale_0, weiscale_0), {args.generations} generations",
                    "plain_lambda": lam, "surrogate_lambda": args.surrogate_lambda, "pool": args.pool,
-                   "learned_fic_start": bool(args.surrogate_fic), "unscreened": args.unscreened},
-        "plain": {"best_cost": best_plain, "full_sims_total": sims_plain_total,
-                  "full_sims_to_best": sims_plain_first, "wall_s": round(w_plain, 1),
-                  "mean_fic_restarts": float(np.mean([float(r["mean_fic_trials"]) for r in plain])),
-                  "best_rerun_plain_cost": plain_rerun},
-        "surrogate": {"best_cost": float(surr[-1]["best_cost"]),
-                      "best_plain_rerun_cost": float(surr[-1]["best_plain_cost"]),
-                      "full_sims_total": int(surr[-1]["full_sims"]),
-                      "wall_s": round(w_surr, 1),
-                      "mean_fic_restarts": float(np.mean([float(r["mean_fic_trials"]) for r in surr])),
-                      "generations_prescreened": int(sum(int(r["surrogate_used"]) for r in surr)),
-                      "rank_corr": [None if r["surrogate_rho"] in ("nan", "") else float(r["surrogate_rho"])
-                                    for r in surr],
-                      "best_rerun_plain_cost": surr_rerun,
-                      "fic_training": surr_text.split("Surrogate FIC initialiser: ")[1].split("\n")[0]
-                      if "Surrogate FIC initialiser: " in surr_text else None},
-        "reduction_vs_plain_to_best": round(sims_plain_first / reach[0], 2) if reach else None,
-        "reduction_vs_plain_total": round(sims_plain_total / reach[0], 2) if reach else None,
-        "progress": {"plain": [(int(r["full_sims"]), float(r["best_plain_cost"])) for r in plain],
-                     "surrogate": [(int(r["full_sims"]), float(r["best_plain_cost"])) for r in surr]},
+                   "learned_fic_start": bool(args.surrogate_fic), "fic_model": bool(args.fic_model),
+                   "guard": bool(args.guard), "unscreened": args.unscreened, "seeds": seeds},
+        "ratios": ratios, "reached": f"{len(ratios)}/{len(rows_out)}",
+        "ratio_min": min(ratios) if ratios else None, "ratio_max": max(ratios) if ratios else None,
+        "seeds": rows_out,
     }
     print(json.dumps(line), flush=True)
+
+
+def trainset_experiment(args):
+    """The learned FIC initialiser's training set (200k FIC-phase simulations
+    of the uneven batch's N = 100 population), the FIC head trained on it on
+    the device, its held-out error, the FIC restarts it saves on fresh sets,
+    and the head written to --fic-model."""
+    from bnm_cuda_amd import trainset as TS
+    t0 = time.perf_counter()
+    meta = TS.write(args.trainset_dir, n_sims=args.trainset_sims, log=log)
+    log(f"[trainset] written: {meta}")
+    head, rep = TS.train(args.trainset_dir, epochs=args.trainset_epochs, log=log)
+    sav = TS.restart_savings(head)
+    log(f"[trainset] restart savings on 4096 fresh sets: {sav}")
+    if args.fic_model:
+        head.save(args.fic_model)
+    head.close()
+    print(json.dumps({"metric": "learned FIC initialiser: training set, fit and FIC restarts saved",
+                      "value": round(sav["mean_restarts_analytical"] / max(sav["mean_restarts_learned"], 1e-9), 3),
+                      "unit": "x fewer FIC restarts (fresh sets, plain budget of 10)", "higher_is_better": True,
+                      "n_gpus": 1, "dtype": "f32 simulation, bf16 MFMA network", "data": "synthetic",
+                      "vs_baseline": None, "trainset": meta, "fit": rep, "restarts": sav,
+                      "total_s": round(time.perf_counter() - t0, 1)}), flush=True)
 
 
 def main():
@@ -273,15 +323,23 @@ def main():
     ap.add_argument("--kernel", default="auto", choices=["auto", "fast", "mfma"],
                     help="integration kernel family of the timed path (profiling variants)")
     ap.add_argument("--workload", default="pop128",
-                    choices=["pop128", "surrogate", "uneven", "fcd400", "scaling", "copy"],
+                    choices=["pop128", "surrogate", "trainset", "uneven", "fcd400", "scaling", "copy"],
                     help="pop128: the headline line; surrogate: CMA-ES with vs without the surrogate; "
                          "uneven: four populations (N 68-400) in one graph launch; fcd400: FC/FCD stress; "
                          "scaling: throughput against population size (N 100, 128-4096 sims)")
     ap.add_argument("--scale", type=float, default=1.0, help="duration scale (uneven workload)")
     ap.add_argument("--generations", type=int, default=32)
-    ap.add_argument("--surrogate-lambda", type=int, default=64)
-    ap.add_argument("--pool", type=int, default=8)
-    ap.add_argument("--surrogate-fic", type=int, default=0, help="learned FIC start in the surrogate run")
+    ap.add_argument("--surrogate-lambda", type=int, default=128)
+    ap.add_argument("--pool", type=int, default=4)
+    ap.add_argument("--surrogate-fic", type=int, default=1, help="learned FIC start in the surrogate run")
+    ap.add_argument("--fic-model", default="", help="FIC head file (trainset: written; surrogate: loaded)")
+    ap.add_argument("--guard", type=int, default=1, help="plain-consistency guard of the learned FIC start")
+    ap.add_argument("--seeds", default="1", help="CMA-ES seeds (comma list) of the surrogate workload")
+    ap.add_argument("--compare-no-fic", type=int, default=1,
+                    help="surrogate workload: also run the pre-screened search without the learned FIC start")
+    ap.add_argument("--trainset-dir", default="/tmp/bnm_fic_trainset")
+    ap.add_argument("--trainset-sims", type=int, default=200_000)
+    ap.add_argument("--trainset-epochs", type=int, default=2)
     ap.add_argument("--unscreened", type=float, default=0.25,
                     help="fraction of each pre-screened generation drawn unscreened (surrogate workload)")
     ap.add_argument("--seed-mw", type=int, default=1)
@@ -297,6 +355,9 @@ def main():
         sys.exit(2)
     if args.workload == "surrogate":
         surrogate_experiment(args)
+        return
+    if args.workload == "trainset":
+        trainset_experiment(args)
         return
     if args.workload == "copy":
         copy_calibration()

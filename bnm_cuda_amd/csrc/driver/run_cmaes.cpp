@@ -17,12 +17,30 @@
 //   BNM_SURR_FIC       0: surrogate search without the learned FIC start
 //
 // Surrogate-guided search (BNM_SURROGATE=1):
-//  * FIC initialiser: before the search, BNM_SURR_FIC_SIMS parameter sets drawn
-//    uniformly from the search box are run through the FIC phase only
-//    (BNM_SURR_FIC_TRIALS restarts allowed); an MLP (inputs G, wEE_j, wEI_j,
-//    SC strength_j, analytical wIE_j; 4 x 256 GELU; BF16 MFMA) learns the
-//    converged wIE_j minus the analytical one.  Every simulation of the search
-//    then starts FIC from analytical + predicted wIE.
+//  * FIC initialiser: an MLP (inputs G, wEE_j, wEI_j, SC strength_j, z_j =
+//    the first map's value at node j, or 0 without maps; 4 x 256 GELU; BF16
+//    MFMA) predicts the converged wIE_j minus the analytical one.  With
+//    BNM_SURR_FIC_MODEL=<file> it is the head trained offline on the
+//    200k-simulation training set (bnm_cuda_amd/trainset.py); otherwise,
+//    before the search, BNM_SURR_FIC_SIMS parameter sets drawn uniformly from
+//    the search box run the FIC phase only (BNM_SURR_FIC_TRIALS restarts
+//    allowed) and train it.  Every simulation of the search then starts FIC
+//    from analytical + predicted wIE.
+//  * online adaptation (BNM_SURR_FIC_ADAPT=1, default): the head is trained
+//    further on the converged wIE of the search's own simulations (the last
+//    BNM_SURR_FIC_WINDOW generations, BNM_SURR_FIC_STEPS Adam steps per
+//    generation); the first generation starts from the analytical wIE, and
+//    a generation uses the learned start only when the head's mean distance
+//    to the converged wIE of the previous generation was below
+//    BNM_SURR_FIC_GATE (0.9) times the analytical start's.
+//  * plain-consistency guard (with the learned start): the jump from the
+//    analytical to the learned start stands for k plain restarts (+init_delta
+//    per restart upwards; init_delta, then 0.001 less per restart, downwards);
+//    a candidate whose learned-start run failed, or whose k plus the restarts
+//    it still took reach the plain budget, might have failed FIC with the
+//    plain method, so it is simulated again with the plain method and that
+//    result is what the search sees (the re-runs are counted as full
+//    simulations).  BNM_SURR_GUARD=0 turns the guard off.
 //  * fit surrogate: an MLP on the unit-box coordinates predicts
 //    (FC r, FCD KS, |mean FC difference|, FIC penalty, cost) of a candidate.
 //    It is trained on the simulations of the last BNM_SURR_WINDOW
@@ -50,6 +68,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
@@ -170,6 +189,56 @@ struct Scaler {
         for (size_t i = 0; i < X.size(); ++i) X[i] = (float)((X[i] - mu[i % d]) / sd[i % d]);
     }
 };
+
+// Plain FIC restarts that move a node's wIE by d (fic_decide, bnm.cpp:239-257):
+// +delta per restart upwards (delta unchanged), delta then delta - 0.001 ...
+// (at least 0.001) per restart downwards.
+int plain_fic_steps(double d, double delta0) {
+    if (d >= 0) return (int)std::ceil(d / delta0 - 1e-12);
+    double moved = 0, delta = delta0;
+    int n = 0;
+    while (moved < -d - 1e-12 && n < 100000) {
+        moved += delta;
+        delta = std::max(delta - 0.001, 0.001);
+        ++n;
+    }
+    return n;
+}
+
+// FIC head file written by bnm_cuda_amd/trainset.py (FICHead.save), little
+// endian: "BNMFIC01", int32 in_dim / hidden / depth / out_dim / bf16, float32
+// mu[in_dim], sd[in_dim], ysd, int64 n_params, float32 params.
+bool load_fic_head(const std::string& path, bnm_mlp** net, Scaler& xs, double* ys, std::string* err) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) { *err = "cannot open " + path; return false; }
+    char magic[8];
+    int32_t h[5];
+    if (!f.read(magic, 8) || std::memcmp(magic, "BNMFIC01", 8) != 0 || !f.read((char*)h, sizeof h)) {
+        *err = path + " is not a FIC head file";
+        return false;
+    }
+    if (h[0] != 5 || h[3] != 1) { *err = "FIC head must map 5 features to 1 output"; return false; }
+    std::vector<float> mu(5), sd(5);
+    float ysd = 1;
+    int64_t n = 0;
+    f.read((char*)mu.data(), 20);
+    f.read((char*)sd.data(), 20);
+    f.read((char*)&ysd, 4);
+    f.read((char*)&n, 8);
+    std::vector<float> p(n > 0 && n < (1LL << 28) ? n : 0);
+    if (!f || p.empty() || !f.read((char*)p.data(), (std::streamsize)(4 * n))) {
+        *err = path + " is truncated";
+        return false;
+    }
+    int st = 0;
+    *net = bnm_mlp_create(5, h[1], h[2], 1, h[4], 7, &st);
+    if (!*net || bnm_mlp_num_params(*net) != n) { *err = "FIC head shape does not match"; return false; }
+    if (bnm_mlp_set_params(*net, p.data())) { *err = bnm_mlp_last_error(); return false; }
+    xs.mu.assign(mu.begin(), mu.end());
+    xs.sd.assign(sd.begin(), sd.end());
+    *ys = ysd;
+    return true;
+}
 
 // Local linear-quadratic cost model for pre-screening (the surrogate of
 // Hansen's lq-CMA-ES, 2019): cost ~ full quadratic in the unit-box
@@ -475,7 +544,10 @@ int main(int argc, char* argv[]) {
         std::vector<int> un(B);
         check(bnm_analytical_fic(nodes, B, SC.data(), G_, ee, ei, out, un.data()), "analytical FIC");
     };
-    auto fic_features = [&](int B, const double* G_, const double* ee, const double* ei, const double* an) {
+    // z_j of the FIC head: the first map (z-scored maps), or 0
+    std::vector<double> zmap(nodes, 0.0);
+    if (!maps.empty()) zmap = maps[0];
+    auto fic_features = [&](int B, const double* G_, const double* ee, const double* ei) {
         std::vector<float> X((size_t)B * nodes * 5);
         for (int b = 0; b < B; ++b)
             for (int j = 0; j < nodes; ++j) {
@@ -485,11 +557,24 @@ int main(int argc, char* argv[]) {
                 r[1] = (float)ee[q];
                 r[2] = (float)ei[q];
                 r[3] = (float)strength[j];
-                r[4] = (float)an[q];
+                r[4] = (float)zmap[j];
             }
         return X;
     };
-    if (surrogate && do_fic && cfg.adjust_fic && env_i("BNM_SURR_FIC", 1)) {
+    const std::string fic_model = env_s("BNM_SURR_FIC_MODEL", "");
+    const bool guard = env_i("BNM_SURR_GUARD", 1) != 0;
+    // online adaptation (BNM_SURR_FIC_ADAPT, on by default): see the loop
+    const bool fic_adapt = env_i("BNM_SURR_FIC_ADAPT", 1) != 0;
+    const int fic_window = std::max(1, env_i("BNM_SURR_FIC_WINDOW", 3));
+    const int fic_steps = env_i("BNM_SURR_FIC_STEPS", 300);
+    const double fic_gate = env_d("BNM_SURR_FIC_GATE", 0.9);
+    std::vector<float> fa_X, fa_Y;
+    std::vector<int> fa_gen;
+    if (surrogate && do_fic && cfg.adjust_fic && env_i("BNM_SURR_FIC", 1) && !fic_model.empty()) {
+        std::string err;
+        if (!load_fic_head(fic_model, &fic_net, fic_xs, &fic_ys, &err)) die("BNM_SURR_FIC_MODEL: " + err);
+        both(log, "Surrogate FIC initialiser: head loaded from %s\n", fic_model.c_str());
+    } else if (surrogate && do_fic && cfg.adjust_fic && env_i("BNM_SURR_FIC", 1)) {
         // FIC-phase training set: parameter sets drawn uniformly in the search box
         const int n_train = env_i("BNM_SURR_FIC_SIMS", 8 * lambda);
         const int t_max = env_i("BNM_SURR_FIC_TRIALS", 50);
@@ -523,7 +608,7 @@ int main(int argc, char* argv[]) {
               "outputs");
         bnm_ctx_destroy(fctx);
         an_fic(n_train, g.data(), ee.data(), ei.data(), an.data());
-        std::vector<float> Xall = fic_features(n_train, g.data(), ee.data(), ei.data(), an.data()), X, Y;
+        std::vector<float> Xall = fic_features(n_train, g.data(), ee.data(), ei.data()), X, Y;
         int rows = 0;
         double tr_sum = 0;
         for (int b = 0; b < n_train; ++b) {
@@ -606,7 +691,7 @@ int main(int argc, char* argv[]) {
         prog = std::fopen((cmaes_dir + "/" + out_prefix + "_progress.csv").c_str(), "w");
         if (prog)
             std::fprintf(prog, "iteration,full_sims,simulated_s,gen_best_cost,best_cost,best_plain_cost,"
-                               "mean_fic_trials,max_fic_trials,surrogate_used,surrogate_rho,gpu_ms\n");
+                               "mean_fic_trials,max_fic_trials,surrogate_used,surrogate_rho,gpu_ms,guard_sims\n");
     }
     // With the learned FIC start every cost the search sees comes from a
     // simulation whose FIC began elsewhere than the reference's analytical
@@ -628,6 +713,9 @@ int main(int argc, char* argv[]) {
     long long full_sims = 0;
     double sim_seconds = 0, last_rho = -1, best_so_far = INFINITY, best_plain = INFINITY;
     bool used_surr = false;
+    // learned FIC start in the current generation (with adaptation the first
+    // generation starts from the analytical wIE and trains the head first)
+    bool fic_on = fic_net && !fic_adapt;
     while (bnm_cmaes_iteration(es) < itMax) {
         const int it = bnm_cmaes_iteration(es);
         both(log, "Iteration %d\n", it);
@@ -676,27 +764,28 @@ int main(int argc, char* argv[]) {
             fill_params(&x[(size_t)b * D], b, G_list.data(), wEE.data(), wEI.data(), wIE.data());
         const double* wfic_in = nullptr;
         std::vector<double> an, init;
+        std::vector<float> fx_raw;
         if (fic_net) {
             an.resize((size_t)lambda * nodes);
             an_fic(lambda, G_list.data(), wEE.data(), wEI.data(), an.data());
-            std::vector<float> X = fic_features(lambda, G_list.data(), wEE.data(), wEI.data(), an.data());
+            fx_raw = fic_features(lambda, G_list.data(), wEE.data(), wEI.data());
+            std::vector<float> X = fx_raw;
             fic_xs.apply(X, 5);
             std::vector<float> y((size_t)lambda * nodes);
             check(bnm_mlp_predict(fic_net, X.data(), lambda * nodes, y.data()), "FIC predict");
             init.resize(an.size());
             for (size_t q = 0; q < an.size(); ++q) init[q] = an[q] + fic_ys * y[q];
-            wfic_in = init.data();
+            if (fic_on) wfic_in = init.data();
         }
         // with the learned initialiser the context takes w_IE_fic as given
-        bnm_ctx* rctx = fic_net ? sctx : ctx;
+        const bool learned = fic_net && fic_on;
+        bnm_ctx* rctx = learned ? sctx : ctx;
         check(bnm_ctx_set_params(rctx, G_list.data(), wEE.data(), wEI.data(), wIE.data(), wfic_in), "params");
         t0 = Clock::now();
         check(bnm_ctx_run(rctx), "run");
         check(bnm_ctx_get_outputs(rctx, corr.data(), pen.data(), wfic.data(), nullptr, nullptr, nullptr, nullptr,
                                   nullptr, nullptr, nullptr, failed.data(), trials.data(), fc.data(), nullptr),
               "outputs");
-        const double took = std::chrono::duration<double>(Clock::now() - t0).count();
-        std::printf("took %lf s\n", took);
         full_sims += lambda;
         double tr_sum = 0;
         int tr_max = 0;
@@ -704,8 +793,97 @@ int main(int argc, char* argv[]) {
             tr_sum += trials[b];
             tr_max = std::max(tr_max, trials[b]);
             sim_seconds += (time_steps + 1 + (do_fic ? (double)trials[b] * (cfg.i_sampling_end + 1) : 0.0)) / 1000.0;
-            raw[b] = corr[b] + (do_fic ? pen[b] : 0.0);   // run_CMAES.cpp:787-793
         }
+        // plain-consistency guard: candidates whose plain FIC might have
+        // failed are simulated again with the plain method (analytical
+        // start) in one batch (the batch's remaining slots repeat them)
+        int n_guard = 0;
+        std::vector<char> guarded(lambda, 0);
+        if (learned && guard) {
+            std::vector<int> flag;
+            for (int b = 0; b < lambda; ++b) {
+                int k = 0;
+                for (int j = 0; j < nodes; ++j) {
+                    const size_t q = (size_t)b * nodes + j;
+                    k = std::max(k, plain_fic_steps(init[q] - an[q], cfg.init_delta));
+                }
+                if (failed[b] || k + trials[b] >= max_trials) flag.push_back(b);
+            }
+            n_guard = (int)flag.size();
+            if (n_guard > 0) {
+                std::vector<double> gG(lambda), gee((size_t)lambda * nodes), gei((size_t)lambda * nodes),
+                    gie((size_t)lambda * nodes), gc(lambda), gp(lambda), gfc((size_t)lambda * n_pairs);
+                std::vector<int> gf(lambda), gt(lambda);
+                std::vector<double> gw((size_t)lambda * nodes);
+                for (int s2 = 0; s2 < lambda; ++s2)
+                    fill_params(&x[(size_t)flag[s2 % n_guard] * D], s2, gG.data(), gee.data(), gei.data(), gie.data());
+                check(bnm_ctx_set_params(ctx, gG.data(), gee.data(), gei.data(), gie.data(), nullptr), "params");
+                check(bnm_ctx_run(ctx), "plain-consistency run");
+                check(bnm_ctx_get_outputs(ctx, gc.data(), gp.data(), gw.data(), nullptr, nullptr, nullptr, nullptr,
+                                          nullptr, nullptr, nullptr, gf.data(), gt.data(), gfc.data(), nullptr),
+                      "outputs");
+                for (int i = 0; i < n_guard; ++i) {
+                    const int b = flag[i];
+                    guarded[b] = 1;
+                    std::copy(&gw[(size_t)i * nodes], &gw[(size_t)(i + 1) * nodes], &wfic[(size_t)b * nodes]);
+                    corr[b] = gc[i];
+                    pen[b] = gp[i];
+                    failed[b] = gf[i];
+                    std::copy(&gfc[(size_t)i * n_pairs], &gfc[(size_t)(i + 1) * n_pairs], &fc[(size_t)b * n_pairs]);
+                    tr_sum += gt[i];
+                    tr_max = std::max(tr_max, gt[i]);
+                    sim_seconds += (time_steps + 1 + (double)gt[i] * (cfg.i_sampling_end + 1)) / 1000.0;
+                }
+                full_sims += n_guard;
+                both(log, "surrogate: plain-consistency re-runs: %d\n", n_guard);
+            }
+        }
+        // online adaptation of the FIC head to this search (SC, regional form):
+        // the converged wIE of this generation's converged simulations join a
+        // window of the last fic_window generations, the head is trained on
+        // it, and the learned start is used in the next generation only if
+        // the head is closer to the converged wIE than the analytical start
+        if (fic_net && fic_adapt) {
+            double e_an = 0, e_lr = 0;
+            int n_ok = 0;
+            for (int b = 0; b < lambda; ++b) {
+                if (failed[b]) continue;
+                for (int j = 0; j < nodes; ++j) {
+                    const size_t q = (size_t)b * nodes + j;
+                    fa_X.insert(fa_X.end(), &fx_raw[q * 5], &fx_raw[q * 5] + 5);
+                    fa_Y.push_back((float)((wfic[q] - an[q]) / fic_ys));
+                    fa_gen.push_back(it);
+                    if (!guarded[b]) {
+                        e_an += std::fabs(an[q] - wfic[q]);
+                        e_lr += std::fabs(init[q] - wfic[q]);
+                    }
+                }
+                if (!guarded[b]) ++n_ok;
+            }
+            // the head's error on this generation, before training on it
+            const bool better = n_ok > 0 && e_lr < fic_gate * e_an;
+            std::vector<float> wx, wy;
+            for (size_t i = 0; i < fa_gen.size(); ++i)
+                if (fa_gen[i] > it - fic_window) {
+                    wx.insert(wx.end(), &fa_X[i * 5], &fa_X[i * 5] + 5);
+                    wy.push_back(fa_Y[i]);
+                }
+            const int nrow = (int)wy.size();
+            if (nrow >= 1024) {
+                fic_xs.apply(wx, 5);
+                check(bnm_mlp_set_data(fic_net, wx.data(), wy.data(), nrow), "FIC data");
+                const int epochs = std::max(1, fic_steps / std::max(1, nrow / 1024));
+                check(bnm_mlp_train(fic_net, epochs, 1024, 3e-4f, 31u + it, nullptr), "FIC adapt");
+            }
+            both(log, "surrogate: FIC start %s this generation; mean |start - converged| analytical %.5f, "
+                      "head %.5f over %d simulations; next generation %s\n",
+                 learned ? "learned" : "analytical", n_ok ? e_an / (n_ok * nodes) : NAN,
+                 n_ok ? e_lr / (n_ok * nodes) : NAN, n_ok, better ? "learned" : "analytical");
+            fic_on = better;
+        }
+        for (int b = 0; b < lambda; ++b) raw[b] = corr[b] + (do_fic ? pen[b] : 0.0);   // run_CMAES.cpp:787-793
+        const double took = std::chrono::duration<double>(Clock::now() - t0).count();
+        std::printf("took %lf s\n", took);
         // surrogate bookkeeping: targets of every simulated candidate
         if (fit_net) {
             std::vector<double> actual(lambda);
@@ -805,9 +983,9 @@ int main(int argc, char* argv[]) {
             best_plain = best_so_far;
         }
         if (prog)
-            std::fprintf(prog, "%d,%lld,%.3f,%.9g,%.9g,%.9g,%.4f,%d,%d,%.4f,%.3f\n", it, full_sims, sim_seconds,
-                         gen_best, best_so_far, best_plain, tr_sum / lambda, tr_max, used_surr ? 1 : 0,
-                         used_surr ? last_rho : NAN, took * 1e3);
+            std::fprintf(prog, "%d,%lld,%.3f,%.9g,%.9g,%.9g,%.4f,%d,%d,%.4f,%.3f,%d\n", it, full_sims, sim_seconds,
+                         gen_best, best_so_far, best_plain, tr_sum / (lambda + n_guard), tr_max, used_surr ? 1 : 0,
+                         used_surr ? last_rho : NAN, took * 1e3, n_guard);
         if (surrogate && used_surr) both(log, "surrogate: rank correlation of predicted and simulated cost %.3f\n", last_rho);
         if (stopped) break;   // early stop, or the covariance lost definiteness
     }
