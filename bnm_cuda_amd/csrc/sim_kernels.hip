@@ -20,6 +20,10 @@ namespace bnm {
 
 namespace {
 
+// buffer offset of lanes without a node: beyond every SC resource (at most
+// (1024 + kSCtPad) * 1024 * 8 bytes), so their loads are dropped
+constexpr int kIdleLaneOffset = 0x40000000;
+
 template <class R> struct Vec4;
 template <> struct Vec4<float> { using T = float4; };
 template <> struct Vec4<double> { using T = double2; };
@@ -939,7 +943,10 @@ __global__ __launch_bounds__(MAXT) void k_sim_hyb64(const double* __restrict__ S
     // read past the array return 0
     const __amdgpu_buffer_rsrc_t scr = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<double*>(SCc), (short)0, (int)((N + (int)kSCtPad) * N * sizeof(double)), 0x00020000);
-    const int voff = jc * CH * (int)sizeof(double);
+    // lanes past the last node take an offset beyond the resource: their
+    // loads return 0 without a memory request (the range check covers the
+    // lane offset, not the chunk base in soffset)
+    const int voff = j < N ? jc * CH * (int)sizeof(double) : kIdleLaneOffset;
     auto chunk = [&](int kb, double* w) {     // the lane's 8 values of chunk kb
         const int so = kb * N * CH * (int)sizeof(double);
 #pragma unroll
@@ -1215,7 +1222,7 @@ __global__ __launch_bounds__(MAXT) void k_sim_hyb(const float* __restrict__ SCc,
     // read past the array return 0
     const __amdgpu_buffer_rsrc_t scr = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<float*>(SCc), (short)0, (int)((N + (int)kSCtPad) * N * sizeof(float)), 0x00020000);
-    const int voff = jc * CH * (int)sizeof(float);
+    const int voff = j < N ? jc * CH * (int)sizeof(float) : kIdleLaneOffset;   // as in k_sim_hyb64
     auto chunk = [&](int kb, float* w) {     // the lane's 16 values of chunk kb
         const int so = kb * N * CH * (int)sizeof(float);
 #pragma unroll
