@@ -127,17 +127,27 @@ def test_driver_matches_library_loop_and_oracle(inputs, oracle_mod, tmp_path):
     np.testing.assert_allclose(np.loadtxt(out / sims[0] / (pre + "w_IE.txt")), o["wIE_fic"][0], atol=1e-6)
 
 
-def test_surrogate_mode_runs_and_reports(inputs, tmp_path):
+@pytest.mark.parametrize("guard", [1, 0])
+def test_surrogate_mode_runs_and_reports(inputs, tmp_path, guard):
+    """Learned FIC start with and without the plain-consistency guard.  Each
+    generation runs its 10 candidates plus the plain re-run of its best; with
+    the guard on, a learned generation also re-runs through the plain FIC
+    every candidate the guard flags, and those count as full simulations."""
     d, *_ = inputs
-    out = tmp_path / "surr"
+    out = tmp_path / f"surr{guard}"
     stdout = _run(d, out, {"BNM_SURROGATE": "1", "BNM_SURR_FIC_SIMS": "64", "BNM_SURR_FIC_EPOCHS": "20",
-                           "BNM_SURR_FIT_STEPS": "300"})
-    # 10 candidates per generation plus the plain re-run of each generation's best
-    assert "Surrogate FIC initialiser" in stdout and "Surrogate summary: full simulations 33" in stdout
+                           "BNM_SURR_FIT_STEPS": "300", "BNM_SURR_GUARD": str(guard)})
+    assert "Surrogate FIC initialiser" in stdout
     assert len(_parse(stdout)) == (ITMAX - 1) * LAM
     prog = [p for p in os.listdir(out / "cmaes_multimaps_gpu") if p.endswith("_progress.csv")][0]
     rows = np.genfromtxt(out / "cmaes_multimaps_gpu" / prog, delimiter=",", names=True)
-    assert rows["full_sims"].tolist() == [11, 22, 33]
+    g = rows["guard_sims"].astype(int)
+    reruns = [int(k) for k in re.findall(r"plain-consistency re-runs: (\d+)", stdout)]
+    assert sum(reruns) == g.sum() and len(reruns) == np.count_nonzero(g)
+    assert g[0] == 0 and (guard or g.sum() == 0) and np.all(g <= LAM)
+    want = np.cumsum(LAM + 1 + g).tolist()
+    assert rows["full_sims"].tolist() == want
+    assert f"Surrogate summary: full simulations {want[-1]}," in stdout
     assert np.all(np.isfinite(rows["best_plain_cost"]))
     assert stdout.count("plain re-run of the generation best") == ITMAX - 1
     assert rows["surrogate_used"].tolist()[0] == 0 and rows["surrogate_used"].max() == 1
