@@ -95,6 +95,12 @@ struct bnm_ctx {
     float* SCt32 = nullptr;
     float* SCc32 = nullptr;   // chunked [k/16][j][k%16] copy for the hybrid kernel
     double* SCc64 = nullptr;  // chunked [k/8][j][k%8] copy for the FP64 hybrid kernel
+    // sparse row lists for the sparse FP32 kernel (N <= 512): weights
+    // [kSparseRW + kSparseLL][N], packed byte steps [that / 4][N]; sp_P is
+    // the slot count of the current SC (0: no SC yet or too dense to fit)
+    float* SPw = nullptr;
+    unsigned* SPd = nullptr;
+    int sp_P = 0;
     size_t lds_max = 64 * 1024;  // dynamic LDS a workgroup may use
     float* noise = nullptr;
     long long noise_cap = 0;
@@ -133,6 +139,8 @@ struct bnm_ctx {
     hipGraphExec_t graph_exec = nullptr;
     bool have_sc = false, have_noise = false, have_emp = false, have_params = false;
     int fast_ls = 0, fast_ks = 0, stream_ks = 0, force_stream = 0, no_ei = 0, ei_ls = 0, ei_bc = 0, ei_form = -1, no_hyb = 0, hyb_ks = 0, hyb_bc = -1, hyb_rr = -1;
+    int no_sparse = 0, sparse_only = 0;
+    int sc_gen = 0;   // bumped when a new SC changes the sparse kernel's shape
     // no-allocation mode (bnm_run_simulations_noalloc): every device buffer is
     // carved out of `arena` (the library's static scratch array), the null
     // stream is used and no events, tuner windows or graphs are created
@@ -320,7 +328,7 @@ int bnm_make_noise_chunked(unsigned seed, long long count, float* out, int threa
 
 void bnm_ctx_destroy(bnm_ctx* x) {
     if (!x) return;
-    void* bufs[] = {x->SC64, x->SC32, x->SCt64, x->SCt32, x->SCc32, x->SCc64, x->noise, x->G, x->wEE, x->wEI, x->wIE, x->wIEfic,
+    void* bufs[] = {x->SC64, x->SC32, x->SCt64, x->SCt32, x->SCc32, x->SCc64, x->SPw, x->SPd, x->noise, x->G, x->wEE, x->wEI, x->wIE, x->wIEfic,
                     x->wIEadj, x->wIEscratch, x->bold, x->ext, x->failed, x->trials, x->unstable,
                     x->seg_start, x->seg_len, x->pidx, x->mean, x->inv, x->wmean, x->winv, x->fc,
                     x->wfc, x->wz, x->wzlo, x->fcd, x->emp_c, x->ks_u, x->ks_fe, x->bins, x->nanf, x->corr, x->pen,
@@ -352,6 +360,9 @@ static int ctx_alloc(bnm_ctx* x) {
     if (x->f64) {
         if ((r = cdalloc(x, &x->SCc64, (N + kSCtPad) * N))) return r;
         HIPCHK(hipMemset(x->SCc64, 0, (N + kSCtPad) * N * sizeof(double)));
+    } else if (N <= 512) {
+        if ((r = cdalloc(x, &x->SPw, (size_t)(kSparseRW + kSparseLL) * N))) return r;
+        if ((r = cdalloc(x, &x->SPd, (size_t)(kSparseRW + kSparseLL) / 4 * N))) return r;
     }
     if ((r = cdalloc(x, &x->G, B))) return r;
     if ((r = cdalloc(x, &x->wEE, B * N))) return r;
@@ -536,6 +547,10 @@ static bnm_ctx* ctx_create(const bnm_config* cfg, int* status, unsigned char* ar
     // BNM_HYB_RR forces the hybrid kernel's register part (values per lane)
     x->hyb_rr = env_int("BNM_HYB_RR", -1);
     x->no_hyb = env_int("BNM_NO_HYB", 0);
+    // BNM_SPARSE=1 pins the sparse FP32 kernel where the SC fits it;
+    // BNM_NO_SPARSE=1 leaves it out of the tuner's candidates
+    x->sparse_only = env_int("BNM_SPARSE", 0);
+    x->no_sparse = env_int("BNM_NO_SPARSE", 0);
     x->hyb_ks = env_int("BNM_HYB_KS", 0);
     {
         int dev = 0, v = 0;
@@ -594,6 +609,27 @@ int bnm_ctx_set_sc(bnm_ctx* x, const double* SC) {
         for (size_t j = 0; j < N; ++j)
             for (size_t k = 0; k < N; ++k) dc[((k / 8) * N + j) * 8 + k % 8] = SC[j * N + k];
         HIPCHK(hipMemcpy(x->SCc64, dc.data(), dc.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    if (x->SPw) {
+        std::vector<float> w;
+        std::vector<unsigned> d;
+        int P = build_sparse_sc(f.data(), (int)N, w, d);
+        if (P > kSparseRW + kSparseLL || P == 0) P = 0;   // too dense for the kernel (or empty)
+        else {
+            HIPCHK(hipMemcpy(x->SPw, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(x->SPd, d.data(), d.size() * sizeof(unsigned), hipMemcpyHostToDevice));
+        }
+        // the slot count is a launch argument and sets the kernel's shape: a
+        // new count re-runs the kernel choice and the graph capture
+        if (P != x->sp_P && x->tuned) {
+            x->tuned = false;
+            if (x->graph_exec) {
+                (void)hipGraphExecDestroy(x->graph_exec);
+                x->graph_exec = nullptr;
+            }
+            ++x->sc_gen;
+        }
+        x->sp_P = P;
     }
     x->have_sc = true;
     return BNM_OK;
@@ -690,6 +726,7 @@ static bool variant_ok(const bnm_ctx* x, const Variant& v) {
         case 5: return !x->f64 && hyb_shape(x->N, v.ks, x->lds_max, x->hyb_rr, v.mode).ok;
         case 6: return x->f64 && ei64_shape(x->N, v.ls).ok;
         case 7: return x->f64 && hyb64_shape(x->N, v.ks, x->lds_max, x->hyb_rr).ok;
+        case 8: return !x->f64 && x->sp_P > 0 && sparse_shape(x->N, x->sp_P, x->lds_max).ok;
         default: return true;
     }
 }
@@ -716,6 +753,9 @@ static std::string variant_name(const bnm_ctx* x, const Variant& v) {
         std::snprintf(name, sizeof name, "hyb_f32(KS=%d,NW=%d,kc=%d,regs=%d,lds=%d,streamed=%d,threads=%d%s)", g.KS,
                       g.NW, g.kc, g.KC, g.SS, g.kc - g.KC - g.SS, g.threads,
                       v.mode == 2 ? ",dpp,pf3" : v.mode == 1 ? ",dpp" : "");
+    } else if (v.kind == 8) {
+        const SparseShape g = sparse_shape(x->N, x->sp_P, x->lds_max);
+        std::snprintf(name, sizeof name, "sparse_f32(P=%d,regs=%d,lds=%d,threads=%d)", g.P, g.RW, g.LL, g.threads);
     } else if (v.kind == 7) {
         const FastShape g = hyb64_shape(x->N, v.ks, x->lds_max, x->hyb_rr);
         std::snprintf(name, sizeof name, "hyb_f64(KS=%d,NW=%d,kc=%d,regs=%d,lds=%d,streamed=%d,threads=%d)", g.KS,
@@ -749,6 +789,7 @@ static hipError_t launch_variant(bnm_ctx* x, const Variant& v, const SimArgs& a,
         case 5: return launch_sim_hyb(x->SCc32, a, v.ks, x->lds_max, x->hyb_rr, v.mode, st);
         case 6: return launch_sim_ei64(x->SC64, a, v.ls, st);
         case 7: return launch_sim_hyb64(x->SCc64, a, v.ks, x->lds_max, x->hyb_rr, st);
+        case 8: return launch_sim_sparse(x->SPw, x->SPd, a, x->sp_P, x->lds_max, st);
         default:
             return x->f64 ? launch_sim_plain<double>(x->SC64, a, st)
                           : launch_sim_plain<float>(x->SC32, a, st);
@@ -772,6 +813,10 @@ static std::vector<Variant> candidates(const bnm_ctx* x) {
     if (c.kernel == BNM_KERNEL_MFMA) {
         add({2, 0, 0, 0});
         return out;
+    }
+    if (x->sparse_only && !x->f64) {
+        add({8, 1, 1, 0});
+        if (!out.empty()) return out;
     }
     if (x->hyb_ks > 0) {
         if (x->f64) add({7, 1, x->hyb_ks, 0});
@@ -825,6 +870,10 @@ static std::vector<Variant> candidates(const bnm_ctx* x) {
                     for (int m : {0, 1, 2})
                         if (x->hyb_bc < 0 || x->hyb_bc == m) add({5, 1, k, m});
         }
+        // sparse row lists (SCs of partial density); beside the streaming
+        // kernels only, whose node update it shares (at N <= 128 the tuned
+        // default is the strict E/I kernel)
+        if (!x->f64 && !x->no_sparse && x->stream_ks <= 0) add({8, 1, 1, 0});
     }
     // the MFMA kernel implements FP32 too; the tuner may pick it for large populations
     if (c.kernel == BNM_KERNEL_AUTO && !x->f64 && !pinned && !x->force_stream) add({2, 0, 0, 0});
@@ -1130,6 +1179,7 @@ struct bnm_multi {
     std::vector<hipEvent_t> joins;
     double ms = 0;
     std::string plan;
+    std::vector<int> sc_gen;   // each context's sc_gen at capture
 };
 
 // Integration time estimate of a tuned context (ms): the tuner's time per
@@ -1253,12 +1303,16 @@ bnm_multi* bnm_multi_create(bnm_ctx** ctxs, int n, int* status) {
     e = hipGraphInstantiate(&m->exec, gph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(gph);
     if (e != hipSuccess) return bad(fail(BNM_E_HIP, "hipGraphInstantiate failed"));
+    for (bnm_ctx* x : m->ctxs) m->sc_gen.push_back(x->sc_gen);
     set(BNM_OK);
     return m;
 }
 
 int bnm_multi_run(bnm_multi* m) {
     if (!m) return fail(BNM_E_ARG, "null handle");
+    for (size_t i = 0; i < m->ctxs.size(); ++i)
+        if (m->ctxs[i]->sc_gen != m->sc_gen[i])
+            return fail(BNM_E_ARG, "a context's new SC changed its kernel shape after capture; create the multi again");
     hipStream_t s0 = m->ctxs[0]->st;
     HIPCHK(hipEventRecord(m->t0, s0));
     HIPCHK(hipGraphLaunch(m->exec, s0));

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstddef>
+#include <vector>
 #include "model.hpp"
 
 namespace bnm {
@@ -39,6 +40,24 @@ FastShape hyb_shape(int N, int KS, size_t lds_bytes, int rr = -1, int form = 0);
 // broadcast with three streamed chunks in flight
 hipError_t launch_sim_hyb(const float* SCc, const SimArgs& a, int KS, size_t lds_bytes, int rr, int form,
                           hipStream_t st);
+// FP32 sparse kernel (N <= 512): each SC row as P slots of (weight, byte
+// step to the next nonzero's S_E), the first kSparseRW weights per lane in
+// registers and up to kSparseLL more in LDS; no SC traffic per substep
+constexpr unsigned kSparseMaxStep = 252;   // 63 nodes: the largest step one byte holds
+constexpr int kSparseRW = 112, kSparseLL = 96;
+constexpr size_t kStaticLds = 1024;   // headroom for a kernel's static LDS
+struct SparseShape {
+    int P;         // slots per row (multiple of 8)
+    int RW;        // register-resident slots (template)
+    int LL;        // LDS-resident slots
+    int threads;
+    size_t smem;
+    bool ok;
+};
+int build_sparse_sc(const float* SC, int N, std::vector<float>& W, std::vector<unsigned>& D);
+SparseShape sparse_shape(int N, int P, size_t lds_bytes);
+hipError_t launch_sim_sparse(const float* W, const unsigned* D, const SimArgs& a, int P, size_t lds_bytes,
+                             hipStream_t st);
 // FP32 split kernel with the node update divided between lanes by population
 FastShape ei_shape(int N, int LS, bool bc = false);
 // FP64 hybrid: register / LDS-resident SC, the rest streamed from the chunked

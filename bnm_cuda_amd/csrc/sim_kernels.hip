@@ -12,6 +12,8 @@
 //                  KS == 1.
 #include <algorithm>
 #include <type_traits>
+#include <utility>
+#include <vector>
 
 #include "kernels.hpp"
 #include "model.hpp"
@@ -375,6 +377,36 @@ __device__ __forceinline__ void bcast_dot(float (&acc)[4], float src, const floa
     if constexpr (NQ >= 8) bcast_fma4<1>(acc, src, sc[Q0 + 4], sc[Q0 + 5], sc[Q0 + 6], sc[Q0 + 7]);
     if constexpr (NQ >= 12) bcast_fma4<2>(acc, src, sc[Q0 + 8], sc[Q0 + 9], sc[Q0 + 10], sc[Q0 + 11]);
     if constexpr (NQ >= 16) bcast_fma4<3>(acc, src, sc[Q0 + 12], sc[Q0 + 13], sc[Q0 + 14], sc[Q0 + 15]);
+}
+
+// a + byte B of d in one VALU instruction (SDWA source select), so that the
+// packed steps stay packed: unpacked, the compiler keeps one register per
+// slot and runs out of them.
+template <int B>
+__device__ __forceinline__ unsigned add_step(unsigned a, unsigned d) {
+    unsigned r;
+    if constexpr (B == 0)
+        asm("v_add_u32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_0"
+            : "=v"(r) : "v"(a), "v"(d));
+    else if constexpr (B == 1)
+        asm("v_add_u32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_1"
+            : "=v"(r) : "v"(a), "v"(d));
+    else if constexpr (B == 2)
+        asm("v_add_u32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_2"
+            : "=v"(r) : "v"(a), "v"(d));
+    else
+        asm("v_add_u32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_3"
+            : "=v"(r) : "v"(a), "v"(d));
+    return r;
+}
+
+// 32-bit LDS byte address of a pointer into shared memory, and a 4-byte LDS
+// read at such an address (one ds_read_b32 on the address register itself)
+__device__ __forceinline__ unsigned lds_byte_addr(const void* p) {
+    return (unsigned)(size_t)(const __attribute__((address_space(3))) void*)p;
+}
+__device__ __forceinline__ float lds_f32(unsigned addr) {
+    return *(const __attribute__((address_space(3))) float*)(size_t)addr;
 }
 
 // FP64 node update of one pool (the FP64 E/I kernel's step, below).
@@ -1401,6 +1433,164 @@ __global__ __launch_bounds__(MAXT) void k_sim_hyb(const float* __restrict__ SCc,
 }
 
 // ---------------------------------------------------------------------------
+// sparse (N <= 512): the SC rows as lists of their nonzeros, for connectomes
+// of partial density (the synthetic 20-40 % SCs).  One lane per node, one
+// workgroup of ceil(N / 64) waves per simulation, one barrier per substep.
+//
+// Row j's nonzeros in increasing k become P slots (P the same for every
+// row): slot p holds the weight w_p and the step d_p = 4 * (k_p - k_{p-1})
+// in bytes (k_{-1} = 0) as one byte, four steps per 32-bit register.  A gap
+// of more than 63 nodes takes zero-weight slots of step 252; a row with
+// fewer nonzeros ends in zero-weight slots of step 0.  The lane walks its row
+// with a running LDS byte address into the S_E buffer (one integer add per
+// slot), gathers S_E[k] with one 4-byte LDS read and multiplies it in.  The
+// first RW weights are register-resident, the remaining P - RW (at most
+// LLMAX) sit in LDS as [slot / 4][N] float4; every step is in registers.
+// Nothing is streamed: at N = 400 and 40 % density (P ~ 196) the whole
+// matrix is on the CU, where the dense FP32 matrix (640 KB) does not fit.
+//
+// A zero weight adds +0 to the dense dot as well (S_E is finite and >= 0),
+// so this kernel and the dense ones differ only in the FP32 grouping of the
+// sum: four partial sums by slot index here, by k in the dense kernels.
+// Lanes past the last node walk node N - 1's row (the same addresses: LDS
+// broadcasts, no extra bank traffic) and keep nothing.
+// ---------------------------------------------------------------------------
+template <int RW, int LLMAX, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_sim_sparse(const float* __restrict__ Wsp, const unsigned* __restrict__ Dsp,
+                                                    SimArgs a, int P) {
+    static_assert(RW % 8 == 0 && LLMAX % 8 == 0, "slots come in groups of eight");
+    constexpr int DW = (RW + LLMAX) / 4;
+    const int sim = blockIdx.x;
+    const int N = a.N;
+    const int j = threadIdx.x;
+    const bool comp = j < N;
+    const int jc = comp ? j : N - 1;
+    const int ldS = (N + 3) & ~3;
+    const int nl4 = P > RW ? (P - RW) / 4 : 0;   // LDS-resident groups of four slots
+    const int nl8 = nl4 / 2;                      // ... of eight (P - RW is a multiple of 8)
+    extern __shared__ unsigned char smem_raw[];
+    float* Sb = reinterpret_cast<float*>(smem_raw);                      // [2][ldS]
+    float4* Wl = reinterpret_cast<float4*>(smem_raw + 8 * (size_t)ldS);  // [nl4][N]
+    const unsigned sbase = lds_byte_addr(Sb);
+    for (int i = j; i < 2 * ldS; i += blockDim.x) Sb[i] = (i % ldS) < N ? 0.001f : 0.0f;
+    // Wsp / Dsp hold kSparseRW + kSparseLL slots (zero past P), so every
+    // build reads them unguarded
+    float w[RW];
+    unsigned dl[DW];
+    const float* wp = Wsp + jc;
+    const unsigned* dp = Dsp + jc;
+#pragma unroll
+    for (int p = 0; p < RW; ++p) w[p] = wp[(size_t)p * N];
+#pragma unroll
+    for (int i = 0; i < DW; ++i) dl[i] = dp[(size_t)i * N];
+    if (comp)
+        for (int c = 0; c < nl4; ++c) {
+            const float* s = Wsp + (size_t)(RW + 4 * c) * N + j;
+            Wl[(size_t)c * N + j] = float4{s[0], s[N], s[2 * (size_t)N], s[3 * (size_t)N]};
+        }
+
+    Lane<float> L;
+    if (comp) load_params(a, sim, j, L);
+    const float gp = gain<float>(a.G[sim]);
+    int ts = 0, vol = 0, trial = 0, buf = 0;
+    bool adjusting = a.adjust_fic != 0, failed = false;
+    const unsigned dur = (unsigned)(a.i_end - a.i_start + 1);
+    const float2* noise2 = reinterpret_cast<const float2*>(a.noise);
+    // noise one substep ahead: substep s loads s + 1's pair, the tenth the
+    // next millisecond's first (reloaded when a FIC restart changes it)
+    long long pre_ms = -1;
+    float2 nzn{0.0f, 0.0f};
+    __syncthreads();
+    while (ts <= a.time_steps) {
+        const long long ms = ts % a.noise_period;
+        const float2* nzp = noise2 + (size_t)ms * 10 * N + j;
+        if (pre_ms != ms && comp) nzn = nzp[0];
+#pragma unroll 1
+        for (int s = 0; s < 10; ++s) {
+            const float2 nz = nzn;
+            if (s < 9) {
+                if (comp) nzn = nzp[(size_t)(s + 1) * N];
+            } else {
+                pre_ms = (ts + 1) % a.noise_period;
+                if (comp) nzn = noise2[(size_t)pre_ms * 10 * N + j];
+            }
+            unsigned addr = sbase + (unsigned)(buf * ldS) * 4u;
+            float acc[4] = {0, 0, 0, 0};
+            // eight slots: the eight addresses (byte b of the packed steps d0,
+            // d1 added in turn), the eight gathers, then the eight products,
+            // so that eight LDS reads are in flight
+            // (each batch its own scheduling region: hoisting later batches'
+            // address chains costs registers the weights need)
+            auto slot8 = [&](const float (&wv)[8], unsigned d0, unsigned d1) {
+                __builtin_amdgcn_sched_barrier(0);
+                unsigned ad[8];
+                ad[0] = add_step<0>(addr, d0);
+                ad[1] = add_step<1>(ad[0], d0);
+                ad[2] = add_step<2>(ad[1], d0);
+                ad[3] = add_step<3>(ad[2], d0);
+                ad[4] = add_step<0>(ad[3], d1);
+                ad[5] = add_step<1>(ad[4], d1);
+                ad[6] = add_step<2>(ad[5], d1);
+                ad[7] = add_step<3>(ad[6], d1);
+                addr = ad[7];
+                float sv[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) sv[q] = lds_f32(ad[q]);
+#pragma unroll
+                for (int q = 0; q < 8; ++q) acc[q & 3] = fmaf(wv[q], sv[q], acc[q & 3]);
+            };
+#pragma unroll
+            for (int p = 0; p < RW; p += 8) {
+                const float wv[8] = {w[p], w[p + 1], w[p + 2], w[p + 3], w[p + 4], w[p + 5], w[p + 6], w[p + 7]};
+                slot8(wv, dl[p >> 2], dl[(p >> 2) + 1]);
+            }
+#pragma unroll
+            for (int c = 0; c < LLMAX / 8; ++c)
+                if (c < nl8) {
+                    const float4 v0 = Wl[(size_t)(2 * c) * N + jc], v1 = Wl[(size_t)(2 * c + 1) * N + jc];
+                    const float wv[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+                    slot8(wv, dl[RW / 4 + 2 * c], dl[RW / 4 + 2 * c + 1]);
+                }
+            const float c = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+            // the dot is complete here, on every lane: left to itself the
+            // compiler sinks the products into the node-lane branch below and
+            // keeps every gathered value live until then
+            asm volatile("" : : "v"(c));
+            if (comp) {
+                Node<float>::step(c, gp, L.wEE, L.wEI, L.wIE, nz.x, nz.y, L.SE, L.SI, L.IE, L.II, L.rE, L.rI);
+                Sb[(buf ^ 1) * ldS + j] = L.SE;
+            }
+            __syncthreads();
+            buf ^= 1;
+        }
+        if (comp) L.bw.step(L.SE);
+        if (ts % a.bold_tr == 0) {
+            if (comp) sample(a, sim, j, vol, L);
+            ++vol;
+        }
+        ++ts;
+        if (adjusting) {
+            if (comp && ts >= a.i_start && ts <= a.i_end) L.mIE += (double)L.IE;
+            if (ts == a.i_end) {
+                const bool out = comp && fic_decide<float>(L.mIE, dur, trial, a.max_fic_trials, L.wIE, L.delta);
+                if (__syncthreads_or(out)) {
+                    if (trial < a.max_fic_trials) {
+                        if (comp) { L.reset(); Sb[buf * ldS + j] = 0.001f; }
+                        ts = 0; vol = 0; ++trial;
+                        __syncthreads();
+                    } else {
+                        failed = true; adjusting = false;
+                    }
+                } else {
+                    adjusting = false;
+                }
+            }
+        }
+    }
+    finish(a, sim, j, comp, vol, trial, failed, L);
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 template <class R>
@@ -1669,6 +1859,84 @@ hipError_t launch_sim_hyb(const float* SCc, const SimArgs& a, int KS, size_t lds
         case 128: launch_hyb<128>(SCc, a, s, form, st); break;
         case 144: launch_hyb<144>(SCc, a, s, form, st); break;
         case 160: launch_hyb<160>(SCc, a, s, form, st); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// Sparse row lists of an N x N row-major FP32 SC (see k_sim_sparse): returns
+// P (a multiple of 8; 0 for a matrix without nonzeros) and, when P fits the
+// kernel (P <= kSparseRW + kSparseLL), fills W[p * N + j] (weights) and
+// D[(p / 4) * N + j] (byte p % 4: the step of slot p, in bytes) for all
+// kSparseRW + kSparseLL slots, zero past P.
+int build_sparse_sc(const float* SC, int N, std::vector<float>& W, std::vector<unsigned>& D) {
+    constexpr size_t kSlots = kSparseRW + kSparseLL;
+    std::vector<std::vector<std::pair<float, unsigned>>> rows(N);
+    size_t P = 0;
+    for (int j = 0; j < N; ++j) {
+        unsigned at = 0;   // byte address of the previous slot's S_E
+        for (int k = 0; k < N; ++k) {
+            const float v = SC[(size_t)j * N + k];
+            if (v == 0.0f) continue;
+            const unsigned to = 4u * (unsigned)k;
+            while (to - at > kSparseMaxStep) {
+                rows[j].push_back({0.0f, kSparseMaxStep});
+                at += kSparseMaxStep;
+            }
+            rows[j].push_back({v, to - at});
+            at = to;
+        }
+        P = std::max(P, rows[j].size());
+    }
+    P = (P + 7) / 8 * 8;
+    if (P > kSlots) return (int)P;   // too many for the kernel; nothing filled
+    W.assign(kSlots * N, 0.0f);
+    D.assign(kSlots / 4 * N, 0u);
+    for (int j = 0; j < N; ++j)
+        for (size_t p = 0; p < rows[j].size(); ++p) {
+            W[p * N + j] = rows[j][p].first;
+            D[p / 4 * N + j] |= rows[j][p].second << (8 * (p % 4));
+        }
+    return (int)P;
+}
+
+// register part: the slot count rounded up to 16 up to kSparseRW, beyond it
+// kSparseRW in registers and the rest (at most kSparseLL) in LDS
+SparseShape sparse_shape(int N, int P, size_t lds_bytes) {
+    SparseShape s{};
+    s.P = P;
+    s.threads = ((N + 63) / 64) * 64;
+    s.RW = P <= kSparseRW ? std::max(16, (P + 15) / 16 * 16) : kSparseRW;
+    s.LL = P > kSparseRW ? P - kSparseRW : 0;
+    const size_t ldS = (size_t)((N + 3) & ~3);
+    s.smem = (2 * ldS + (size_t)s.LL * N) * sizeof(float);
+    // (kStaticLds: the workgroup reductions' own LDS beside the dynamic part)
+    s.ok = N >= 1 && s.threads <= 512 && P > 0 && P % 8 == 0 && s.LL <= kSparseLL &&
+           s.smem + kStaticLds <= lds_bytes;
+    return s;
+}
+
+template <int RW, int LLMAX>
+static void launch_sparse(const float* W, const unsigned* D, const SimArgs& a, const SparseShape& s, hipStream_t st) {
+    hipLaunchKernelGGL((k_sim_sparse<RW, LLMAX, 512>), dim3(a.B), dim3(s.threads), s.smem, st, W, D, a, s.P);
+}
+
+hipError_t launch_sim_sparse(const float* W, const unsigned* D, const SimArgs& a, int P, size_t lds_bytes,
+                             hipStream_t st) {
+    const SparseShape s = sparse_shape(a.N, P, lds_bytes);
+    if (!s.ok) return hipErrorInvalidValue;
+    static_assert(kSparseRW == 112 && kSparseLL == 96, "instantiations below");
+    switch (s.RW) {
+        case 16: launch_sparse<16, 0>(W, D, a, s, st); break;
+        case 32: launch_sparse<32, 0>(W, D, a, s, st); break;
+        case 48: launch_sparse<48, 0>(W, D, a, s, st); break;
+        case 64: launch_sparse<64, 0>(W, D, a, s, st); break;
+        case 80: launch_sparse<80, 0>(W, D, a, s, st); break;
+        case 96: launch_sparse<96, 0>(W, D, a, s, st); break;
+        case 112:
+            if (s.LL > 0) launch_sparse<112, 96>(W, D, a, s, st);
+            else launch_sparse<112, 0>(W, D, a, s, st);
+            break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -30,12 +30,12 @@ def _population(N, B, T, seed, precision=bnm.PREC_F64, **kw):
     return cfg, SC, (G, wEE, wEI), noise, emp
 
 
-def _ctx(cfg, SC, params, noise, emp):
+def _ctx(cfg, SC, params, noise, emp, wIE_fic=None):
     c = bnm.Context(cfg)
     c.set_sc(SC)
     c.set_noise(noise)
     c.set_empirical(*emp)
-    c.set_params(*params)
+    c.set_params(*params, wIE_fic=wIE_fic)
     return c
 
 
@@ -83,6 +83,32 @@ def test_graph_replay_matches_eager(precision, N, T):
         eager.run()                      # eager work between the replays
         _same(eager.outputs(), ref)
     assert gctx.timings()["total"] > 0
+    gctx.close()
+    eager.close()
+
+
+@pytest.mark.parametrize("precision", [bnm.PREC_F64, bnm.PREC_F32])
+def test_graph_replay_with_caller_fic_start_is_idempotent(precision):
+    """analytical_fic = 0: numerical FIC starts from the caller's w_IE_fic.
+    The adjusted wIE goes to its own buffer, so every graph replay and every
+    eager re-run starts from the caller's values and gives the same bits."""
+    print("case: graph replay caller FIC start, precision", precision, flush=True)
+    cfg, SC, params, noise, emp = _population(20, 6, 40_000, 3, precision, autotune=0)
+    cfg.analytical_fic = 0
+    an, _ = bnm.analytical_fic(SC, *params)
+    start = an * (1 + 0.05 * np.random.default_rng(3).normal(size=an.shape))
+    eager = _ctx(cfg, SC, params, noise, emp, wIE_fic=start)
+    eager.run()
+    ref = eager.outputs()
+    assert ref["fic_trials"].max() > 0 and np.abs(ref["wIE_fic"] - start).max() > 1e-6
+    gcfg = bnm.Config.from_buffer_copy(cfg)
+    gcfg.use_graph = 1
+    gctx = _ctx(gcfg, SC, params, noise, emp, wIE_fic=start)
+    for rep in range(3):
+        gctx.run()
+        _same(gctx.outputs(), ref)
+        eager.run()
+        _same(eager.outputs(), ref)
     gctx.close()
     eager.close()
 

@@ -1,0 +1,186 @@
+"""The sparse FP32 integration kernel (SC rows as lists of their nonzeros, for
+N > 128 connectomes of partial density such as the synthetic 20-40 % SCs of
+the uneven-batch workload) against the CPU oracle, and its edge cases.
+
+Parity: N = 400 at 40 % density (about 200 slots per row: 112 register- and
+88 LDS-resident weights) and N = 200 at 33 % (88 slots, all in registers),
+FIC on, against the oracle's FP64 run: identical FIC trials, mean r_E within
+0.05 Hz, FC within 1e-3, FCD-KS within 0.01, per-TR BOLD within 1e-5
+relative (measured: 6.5e-7).  The sparse sum adds the same nonzero products
+as the dense one in another FP32 grouping, so against the dense hybrid
+kernel it agrees to FP32 rounding.
+
+Edge cases: a block-structured SC with isolated nodes (empty rows) and gaps
+of more than 63 nodes between a row's nonzeros (zero-weight step slots); an
+SC too dense for the kernel (it is not offered, and a pinned request falls
+back); a new SC with another slot count after the kernel choice (re-tuned;
+a Multi captured before refuses to run); graph replays of the sparse kernel.
+"""
+import numpy as np
+import pytest
+
+import bnm_cuda_amd as bnm
+from bnm_cuda_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+WS, STEP = 4, 1
+
+
+def _floor(p, floor=0.001):
+    m = p.min(axis=1, keepdims=True)
+    return np.where(m < floor, p - (m - floor), p)
+
+
+def _pop(N, SC, B, T, seed):
+    rng = np.random.default_rng(seed)
+    g, ee, ei = W.population(B, seed)
+    z = rng.normal(0, 1, N)
+    return dict(N=N, B=B, T=T, SC=SC, G=g, wEE=_floor(W.heterogeneous(ee, z, 0.1)),
+                wEI=_floor(W.heterogeneous(ei, z, 0.1)), noise=W.noise(N, T + 1, seed + 1),
+                emp_fc=rng.uniform(-0.2, 0.8, bnm.n_pairs(N, True)), emp_fcd=rng.uniform(-0.5, 1.0, 40))
+
+
+def _cfg(p, precision=bnm.PREC_F32, **kw):
+    return bnm.default_config(nodes=p["N"], n_sims=p["B"], time_steps=p["T"], bold_tr=1000, n_vols_remove=0,
+                              window_size=WS, window_step=STEP, noise_period_ms=p["T"] + 1, max_fic_trials=2,
+                              precision=precision,
+                              fc_precision=bnm.FC_F64 if precision == bnm.PREC_F64 else bnm.FC_F32, **kw)
+
+
+def _ctx(p, cfg):
+    c = bnm.Context(cfg)
+    c.set_sc(p["SC"])
+    c.set_noise(p["noise"])
+    c.set_empirical(p["emp_fc"], p["emp_fcd"])
+    c.set_params(p["G"], p["wEE"], p["wEI"])
+    return c
+
+
+def _run(p, monkeypatch, env, **kw):
+    for k in ("BNM_SPARSE", "BNM_NO_SPARSE"):
+        monkeypatch.delenv(k, raising=False)
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    with _ctx(p, _cfg(p, **kw)) as c:
+        c.run()
+        return c.outputs(), c.kernel_name()
+
+
+def _oracle(oracle_mod, p):
+    return oracle_mod.run_simulations(p["SC"], p["G"], p["wEE"], p["wEI"], noise_arr=p["noise"],
+                                      noise_period_ms=p["T"] + 1, time_steps=p["T"], bold_tr=1000,
+                                      n_vols_remove=0, window_size=WS, window_step=STEP, emp_fc=p["emp_fc"],
+                                      emp_fcd=p["emp_fcd"], max_fic_trials=2)
+
+
+def _check_summary(oracle_mod, p, g, r, bold_rel=1e-5):
+    d_re = np.abs(g["ext"][:, 2] - r["ext"][:, 2]).max()
+    d_fc = np.abs(g["fc"] - r["fc"]).max()
+    d_ks = max(abs(oracle_mod.ks(p["emp_fcd"], g["fcd"][b]) - oracle_mod.ks(p["emp_fcd"], r["fcd"][b]))
+               for b in range(p["B"]))
+    rel = max(np.abs(g["bold"][b] - r["bold"][b]).max() / np.abs(r["bold"][b]).max() for b in range(p["B"]))
+    print(f"trials {g['fic_trials']} / {r['fic_trials']}, d mean rE {d_re:.3g} Hz, d FC {d_fc:.3g}, "
+          f"d KS {d_ks:.3g}, per-TR BOLD rel {rel:.3g}", flush=True)
+    np.testing.assert_array_equal(g["fic_trials"], r["fic_trials"])
+    assert d_re <= 0.05 and d_fc <= 1e-3 and d_ks <= 0.01
+    assert rel <= bold_rel
+
+
+@pytest.mark.parametrize("N,seed,dens,want", [(400, 13, 0.40, "regs=112,lds=8"), (200, 12, 0.33, "lds=0")])
+def test_sparse_kernel_matches_oracle(oracle_mod, monkeypatch, N, seed, dens, want):
+    print(f"case: sparse N={N} density {dens}", flush=True)
+    p = _pop(N, W.synthetic_sc(N, dens, seed), 3, 10_000, seed)
+    g, name = _run(p, monkeypatch, {"BNM_SPARSE": "1"})
+    print(name, flush=True)
+    assert name.startswith("sparse_f32") and want in name
+    _check_summary(oracle_mod, p, g, _oracle(oracle_mod, p))
+    # the dense hybrid kernel on the same population: FP32 rounding apart
+    h, hname = _run(p, monkeypatch, {"BNM_NO_SPARSE": "1"})
+    print(hname, "| d BOLD vs dense", np.abs(g["bold"] - h["bold"]).max(), flush=True)
+    np.testing.assert_array_equal(g["fic_trials"], h["fic_trials"])
+    assert np.abs(g["fc"] - h["fc"]).max() <= 1e-4
+
+
+def _block_sc(N=300, seed=21):
+    """Two dense-ish blocks of 140 nodes (30 % inside), 20 isolated nodes at
+    the end (empty rows and columns), and a few links from the first nodes
+    to the second block: gaps of more than 63 nodes inside rows."""
+    rng = np.random.default_rng(seed)
+    A = np.zeros((N, N))
+    for lo in (0, 140):
+        blk = (rng.random((140, 140)) < 0.3) * rng.lognormal(0, 1, (140, 140))
+        A[lo:lo + 140, lo:lo + 140] = np.triu(blk, 1)
+    for i in range(10):
+        A[i, 270 - 2 * i] = 0.5
+    A = A + A.T
+    return A / A.sum(1).mean()
+
+
+def test_sparse_kernel_long_gaps_and_empty_rows(oracle_mod, monkeypatch):
+    print("case: sparse block SC N=300", flush=True)
+    SC = _block_sc()
+    assert np.all(SC[280:] == 0) and np.any(np.diff(np.nonzero(SC[0])[0]) > 63)
+    p = _pop(300, SC, 2, 8_000, 31)
+    g, name = _run(p, monkeypatch, {"BNM_SPARSE": "1"})
+    print(name, flush=True)
+    assert name.startswith("sparse_f32")
+    _check_summary(oracle_mod, p, g, _oracle(oracle_mod, p))
+
+
+def test_sparse_kernel_not_offered_for_a_dense_sc(monkeypatch):
+    print("case: sparse dense SC N=300", flush=True)
+    p = _pop(300, W.synthetic_sc(300, 0.9, 5), 2, 6_000, 5)
+    _, name = _run(p, monkeypatch, {"BNM_SPARSE": "1"})
+    print(name, flush=True)
+    assert not name.startswith("sparse")
+
+
+def test_new_sc_with_another_slot_count_retunes(monkeypatch):
+    """A context tuned on one SC, given an SC with another slot count,
+    chooses again and matches a fresh context; a Multi captured before the
+    change refuses to run instead of launching the old shape."""
+    print("case: sparse SC change N=260", flush=True)
+    monkeypatch.setenv("BNM_SPARSE", "1")
+    monkeypatch.delenv("BNM_NO_SPARSE", raising=False)
+    p1 = _pop(260, W.synthetic_sc(260, 0.25, 41), 2, 6_000, 41)
+    p2 = dict(p1, SC=W.synthetic_sc(260, 0.40, 42))
+    c = _ctx(p1, _cfg(p1))
+    c.run()
+    n1 = c.kernel_name()
+    m = bnm.Multi([c])
+    m.run()
+    c.set_sc(p2["SC"])
+    c.run()
+    n2 = c.kernel_name()
+    print(n1, "->", n2, flush=True)
+    assert n1 != n2 and n1.startswith("sparse_f32") and n2.startswith("sparse_f32")
+    with _ctx(p2, _cfg(p2)) as fresh:
+        fresh.run()
+        a, b = c.outputs(), fresh.outputs()
+        for k in ("bold", "fc", "fcd", "fic_trials", "wIE_fic"):
+            np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+    with pytest.raises(bnm.BnmError):
+        m.run()
+    m.close()
+    c.close()
+
+
+def test_sparse_kernel_graph_replays(monkeypatch):
+    print("case: sparse graph replay N=330", flush=True)
+    monkeypatch.setenv("BNM_SPARSE", "1")
+    monkeypatch.delenv("BNM_NO_SPARSE", raising=False)
+    p = _pop(330, W.synthetic_sc(330, 0.3, 51), 3, 6_000, 51)
+    eager = _ctx(p, _cfg(p))
+    eager.run()
+    ref = eager.outputs()
+    assert eager.kernel_name().startswith("sparse_f32")
+    gctx = _ctx(p, _cfg(p, use_graph=1))
+    for rep in range(3):
+        gctx.run()
+        o = gctx.outputs()
+        for k in ("bold", "fc", "fcd", "fic_trials", "wIE_fic", "ext", "corr"):
+            np.testing.assert_array_equal(o[k], ref[k], err_msg=k)
+        eager.run()
+    gctx.close()
+    eager.close()
