@@ -95,12 +95,14 @@ struct bnm_ctx {
     float* SCt32 = nullptr;
     float* SCc32 = nullptr;   // chunked [k/16][j][k%16] copy for the hybrid kernel
     double* SCc64 = nullptr;  // chunked [k/8][j][k%8] copy for the FP64 hybrid kernel
-    // sparse row lists for the sparse FP32 kernel (N <= 512): weights
-    // [kSparseRW + kSparseLL][N], packed byte steps [that / 4][N]; sp_P is
-    // the slot count of the current SC (0: no SC yet or too dense to fit)
-    float* SPw = nullptr;
-    unsigned* SPd = nullptr;
-    int sp_P = 0;
+    // sparse row lists for the sparse FP32 kernel (N <= 512), one lane per
+    // node ([0]) and two (each half the row's k-range, N <= 256, [1]):
+    // weights [kSparseRW + kSparseLL][LS * N], packed byte steps [that / 4]
+    // [LS * N]; sp_P the slots per lane of the current SC (0: no SC yet, or
+    // too dense to fit)
+    float* SPw[2] = {nullptr, nullptr};
+    unsigned* SPd[2] = {nullptr, nullptr};
+    int sp_P[2] = {0, 0};
     size_t lds_max = 64 * 1024;  // dynamic LDS a workgroup may use
     float* noise = nullptr;
     long long noise_cap = 0;
@@ -328,7 +330,7 @@ int bnm_make_noise_chunked(unsigned seed, long long count, float* out, int threa
 
 void bnm_ctx_destroy(bnm_ctx* x) {
     if (!x) return;
-    void* bufs[] = {x->SC64, x->SC32, x->SCt64, x->SCt32, x->SCc32, x->SCc64, x->SPw, x->SPd, x->noise, x->G, x->wEE, x->wEI, x->wIE, x->wIEfic,
+    void* bufs[] = {x->SC64, x->SC32, x->SCt64, x->SCt32, x->SCc32, x->SCc64, x->SPw[0], x->SPw[1], x->SPd[0], x->SPd[1], x->noise, x->G, x->wEE, x->wEI, x->wIE, x->wIEfic,
                     x->wIEadj, x->wIEscratch, x->bold, x->ext, x->failed, x->trials, x->unstable,
                     x->seg_start, x->seg_len, x->pidx, x->mean, x->inv, x->wmean, x->winv, x->fc,
                     x->wfc, x->wz, x->wzlo, x->fcd, x->emp_c, x->ks_u, x->ks_fe, x->bins, x->nanf, x->corr, x->pen,
@@ -360,9 +362,13 @@ static int ctx_alloc(bnm_ctx* x) {
     if (x->f64) {
         if ((r = cdalloc(x, &x->SCc64, (N + kSCtPad) * N))) return r;
         HIPCHK(hipMemset(x->SCc64, 0, (N + kSCtPad) * N * sizeof(double)));
-    } else if (N <= 512) {
-        if ((r = cdalloc(x, &x->SPw, (size_t)(kSparseRW + kSparseLL) * N))) return r;
-        if ((r = cdalloc(x, &x->SPd, (size_t)(kSparseRW + kSparseLL) / 4 * N))) return r;
+    } else {
+        for (int i = 0; i < 2; ++i) {
+            const size_t ls = i + 1;
+            if (N > 512 / ls) continue;
+            if ((r = cdalloc(x, &x->SPw[i], (size_t)(kSparseRW + kSparseLL) * ls * N))) return r;
+            if ((r = cdalloc(x, &x->SPd[i], (size_t)(kSparseRW + kSparseLL) / 4 * ls * N))) return r;
+        }
     }
     if ((r = cdalloc(x, &x->G, B))) return r;
     if ((r = cdalloc(x, &x->wEE, B * N))) return r;
@@ -547,8 +553,9 @@ static bnm_ctx* ctx_create(const bnm_config* cfg, int* status, unsigned char* ar
     // BNM_HYB_RR forces the hybrid kernel's register part (values per lane)
     x->hyb_rr = env_int("BNM_HYB_RR", -1);
     x->no_hyb = env_int("BNM_NO_HYB", 0);
-    // BNM_SPARSE=1 pins the sparse FP32 kernel where the SC fits it;
-    // BNM_NO_SPARSE=1 leaves it out of the tuner's candidates
+    // BNM_SPARSE pins the sparse FP32 kernel where the SC fits it (1: one
+    // lane per node, 2: two, 3: either, tuned); BNM_NO_SPARSE=1 leaves it out
+    // of the tuner's candidates
     x->sparse_only = env_int("BNM_SPARSE", 0);
     x->no_sparse = env_int("BNM_NO_SPARSE", 0);
     x->hyb_ks = env_int("BNM_HYB_KS", 0);
@@ -610,18 +617,23 @@ int bnm_ctx_set_sc(bnm_ctx* x, const double* SC) {
             for (size_t k = 0; k < N; ++k) dc[((k / 8) * N + j) * 8 + k % 8] = SC[j * N + k];
         HIPCHK(hipMemcpy(x->SCc64, dc.data(), dc.size() * sizeof(double), hipMemcpyHostToDevice));
     }
-    if (x->SPw) {
+    int sp[2] = {0, 0};
+    for (int i = 0; i < 2; ++i) {
+        if (!x->SPw[i]) continue;
         std::vector<float> w;
         std::vector<unsigned> d;
-        int P = build_sparse_sc(f.data(), (int)N, w, d);
+        int P = build_sparse_sc(f.data(), (int)N, i + 1, w, d);
         if (P > kSparseRW + kSparseLL || P == 0) P = 0;   // too dense for the kernel (or empty)
         else {
-            HIPCHK(hipMemcpy(x->SPw, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(x->SPd, d.data(), d.size() * sizeof(unsigned), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(x->SPw[i], w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(x->SPd[i], d.data(), d.size() * sizeof(unsigned), hipMemcpyHostToDevice));
         }
-        // the slot count is a launch argument and sets the kernel's shape: a
+        sp[i] = P;
+    }
+    {
+        // the slot counts are launch arguments and set the kernel's shape: a
         // new count re-runs the kernel choice and the graph capture
-        if (P != x->sp_P && x->tuned) {
+        if ((sp[0] != x->sp_P[0] || sp[1] != x->sp_P[1]) && x->tuned) {
             x->tuned = false;
             if (x->graph_exec) {
                 (void)hipGraphExecDestroy(x->graph_exec);
@@ -629,7 +641,8 @@ int bnm_ctx_set_sc(bnm_ctx* x, const double* SC) {
             }
             ++x->sc_gen;
         }
-        x->sp_P = P;
+        x->sp_P[0] = sp[0];
+        x->sp_P[1] = sp[1];
     }
     x->have_sc = true;
     return BNM_OK;
@@ -726,7 +739,9 @@ static bool variant_ok(const bnm_ctx* x, const Variant& v) {
         case 5: return !x->f64 && hyb_shape(x->N, v.ks, x->lds_max, x->hyb_rr, v.mode).ok;
         case 6: return x->f64 && ei64_shape(x->N, v.ls).ok;
         case 7: return x->f64 && hyb64_shape(x->N, v.ks, x->lds_max, x->hyb_rr).ok;
-        case 8: return !x->f64 && x->sp_P > 0 && sparse_shape(x->N, x->sp_P, x->lds_max).ok;
+        case 8:
+            return !x->f64 && (v.ls == 1 || v.ls == 2) && x->sp_P[v.ls - 1] > 0 &&
+                   sparse_shape(x->N, x->sp_P[v.ls - 1], v.ls, x->lds_max).ok;
         default: return true;
     }
 }
@@ -754,8 +769,9 @@ static std::string variant_name(const bnm_ctx* x, const Variant& v) {
                       g.NW, g.kc, g.KC, g.SS, g.kc - g.KC - g.SS, g.threads,
                       v.mode == 2 ? ",dpp,pf3" : v.mode == 1 ? ",dpp" : "");
     } else if (v.kind == 8) {
-        const SparseShape g = sparse_shape(x->N, x->sp_P, x->lds_max);
-        std::snprintf(name, sizeof name, "sparse_f32(P=%d,regs=%d,lds=%d,threads=%d)", g.P, g.RW, g.LL, g.threads);
+        const SparseShape g = sparse_shape(x->N, x->sp_P[v.ls - 1], v.ls, x->lds_max);
+        std::snprintf(name, sizeof name, "sparse_f32(LS=%d,P=%d,regs=%d,lds=%d,threads=%d)", g.LS, g.P, g.RW, g.LL,
+                      g.threads);
     } else if (v.kind == 7) {
         const FastShape g = hyb64_shape(x->N, v.ks, x->lds_max, x->hyb_rr);
         std::snprintf(name, sizeof name, "hyb_f64(KS=%d,NW=%d,kc=%d,regs=%d,lds=%d,streamed=%d,threads=%d)", g.KS,
@@ -789,7 +805,7 @@ static hipError_t launch_variant(bnm_ctx* x, const Variant& v, const SimArgs& a,
         case 5: return launch_sim_hyb(x->SCc32, a, v.ks, x->lds_max, x->hyb_rr, v.mode, st);
         case 6: return launch_sim_ei64(x->SC64, a, v.ls, st);
         case 7: return launch_sim_hyb64(x->SCc64, a, v.ks, x->lds_max, x->hyb_rr, st);
-        case 8: return launch_sim_sparse(x->SPw, x->SPd, a, x->sp_P, x->lds_max, st);
+        case 8: return launch_sim_sparse(x->SPw[v.ls - 1], x->SPd[v.ls - 1], a, x->sp_P[v.ls - 1], v.ls, x->lds_max, st);
         default:
             return x->f64 ? launch_sim_plain<double>(x->SC64, a, st)
                           : launch_sim_plain<float>(x->SC32, a, st);
@@ -815,7 +831,8 @@ static std::vector<Variant> candidates(const bnm_ctx* x) {
         return out;
     }
     if (x->sparse_only && !x->f64) {
-        add({8, 1, 1, 0});
+        if (x->sparse_only & 1) add({8, 1, 1, 0});
+        if (x->sparse_only & 2) add({8, 2, 1, 0});
         if (!out.empty()) return out;
     }
     if (x->hyb_ks > 0) {
@@ -873,7 +890,10 @@ static std::vector<Variant> candidates(const bnm_ctx* x) {
         // sparse row lists (SCs of partial density); beside the streaming
         // kernels only, whose node update it shares (at N <= 128 the tuned
         // default is the strict E/I kernel)
-        if (!x->f64 && !x->no_sparse && x->stream_ks <= 0) add({8, 1, 1, 0});
+        if (!x->f64 && !x->no_sparse && x->stream_ks <= 0) {
+            add({8, 1, 1, 0});
+            add({8, 2, 1, 0});
+        }
     }
     // the MFMA kernel implements FP32 too; the tuner may pick it for large populations
     if (c.kernel == BNM_KERNEL_AUTO && !x->f64 && !pinned && !x->force_stream) add({2, 0, 0, 0});

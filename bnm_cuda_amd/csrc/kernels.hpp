@@ -47,16 +47,17 @@ constexpr unsigned kSparseMaxStep = 252;   // 63 nodes: the largest step one byt
 constexpr int kSparseRW = 112, kSparseLL = 96;
 constexpr size_t kStaticLds = 1024;   // headroom for a kernel's static LDS
 struct SparseShape {
-    int P;         // slots per row (multiple of 8)
+    int P;         // slots per lane (multiple of 8)
+    int LS;        // lanes per node (1, or 2: each takes half the row's k-range)
     int RW;        // register-resident slots (template)
     int LL;        // LDS-resident slots
     int threads;
     size_t smem;
     bool ok;
 };
-int build_sparse_sc(const float* SC, int N, std::vector<float>& W, std::vector<unsigned>& D);
-SparseShape sparse_shape(int N, int P, size_t lds_bytes);
-hipError_t launch_sim_sparse(const float* W, const unsigned* D, const SimArgs& a, int P, size_t lds_bytes,
+int build_sparse_sc(const float* SC, int N, int LS, std::vector<float>& W, std::vector<unsigned>& D);
+SparseShape sparse_shape(int N, int P, int LS, size_t lds_bytes);
+hipError_t launch_sim_sparse(const float* W, const unsigned* D, const SimArgs& a, int P, int LS, size_t lds_bytes,
                              hipStream_t st);
 // FP32 split kernel with the node update divided between lanes by population
 FastShape ei_shape(int N, int LS, bool bc = false);

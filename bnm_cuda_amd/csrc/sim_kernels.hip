@@ -1441,12 +1441,14 @@ __global__ __launch_bounds__(MAXT) void k_sim_hyb(const float* __restrict__ SCc,
 // row): slot p holds the weight w_p and the step d_p = 4 * (k_p - k_{p-1})
 // in bytes (k_{-1} = 0) as one byte, four steps per 32-bit register.  A gap
 // of more than 63 nodes takes zero-weight slots of step 252; a row with
-// fewer nonzeros ends in zero-weight slots of step 0.  The lane walks its row
+// fewer nonzeros gets zero-weight slots of step 0 spread evenly between its
+// own (so that all lanes walk k at one pace and their gathers fall in
+// distinct LDS banks more often; build_sparse_sc).  The lane walks its row
 // with a running LDS byte address into the S_E buffer (one integer add per
 // slot), gathers S_E[k] with one 4-byte LDS read and multiplies it in.  The
 // first RW weights are register-resident, the remaining P - RW (at most
 // LLMAX) sit in LDS as [slot / 4][N] float4; every step is in registers.
-// Nothing is streamed: at N = 400 and 40 % density (P ~ 196) the whole
+// Nothing is streamed: at N = 400 and 40 % density (P ~ 200) the whole
 // matrix is on the CU, where the dense FP32 matrix (640 KB) does not fit.
 //
 // A zero weight adds +0 to the dense dot as well (S_E is finite and >= 0),
@@ -1455,38 +1457,45 @@ __global__ __launch_bounds__(MAXT) void k_sim_hyb(const float* __restrict__ SCc,
 // Lanes past the last node walk node N - 1's row (the same addresses: LDS
 // broadcasts, no extra bank traffic) and keep nothing.
 // ---------------------------------------------------------------------------
-template <int RW, int LLMAX, int MAXT>
+template <int RW, int LLMAX, int MAXT, int LS>
 __global__ __launch_bounds__(MAXT) void k_sim_sparse(const float* __restrict__ Wsp, const unsigned* __restrict__ Dsp,
                                                     SimArgs a, int P) {
     static_assert(RW % 8 == 0 && LLMAX % 8 == 0, "slots come in groups of eight");
+    static_assert(LS == 1 || LS == 2, "one or two lanes per node");
     constexpr int DW = (RW + LLMAX) / 4;
+    constexpr int GN = 64 / LS;   // nodes per wave
     const int sim = blockIdx.x;
     const int N = a.N;
-    const int j = threadIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int sl = lane / GN;                      // the lane's slice of the row
+    const int j = wave * GN + lane % GN;
     const bool comp = j < N;
+    const bool writer = comp && sl == 0;           // publishes S_E and the outputs
     const int jc = comp ? j : N - 1;
+    const size_t li = (size_t)sl * N + jc;         // the lane's column in W / D / LDS weights
     const int ldS = (N + 3) & ~3;
+    const int KH = (N + LS - 1) / LS;              // nodes per slice
     const int nl4 = P > RW ? (P - RW) / 4 : 0;   // LDS-resident groups of four slots
     const int nl8 = nl4 / 2;                      // ... of eight (P - RW is a multiple of 8)
     extern __shared__ unsigned char smem_raw[];
     float* Sb = reinterpret_cast<float*>(smem_raw);                      // [2][ldS]
-    float4* Wl = reinterpret_cast<float4*>(smem_raw + 8 * (size_t)ldS);  // [nl4][N]
+    float4* Wl = reinterpret_cast<float4*>(smem_raw + 8 * (size_t)ldS);  // [nl4][LS * N]
     const unsigned sbase = lds_byte_addr(Sb);
-    for (int i = j; i < 2 * ldS; i += blockDim.x) Sb[i] = (i % ldS) < N ? 0.001f : 0.0f;
+    for (int i = threadIdx.x; i < 2 * ldS; i += blockDim.x) Sb[i] = (i % ldS) < N ? 0.001f : 0.0f;
     // Wsp / Dsp hold kSparseRW + kSparseLL slots (zero past P), so every
     // build reads them unguarded
     float w[RW];
     unsigned dl[DW];
-    const float* wp = Wsp + jc;
-    const unsigned* dp = Dsp + jc;
+    const float* wp = Wsp + li;
+    const unsigned* dp = Dsp + li;
 #pragma unroll
-    for (int p = 0; p < RW; ++p) w[p] = wp[(size_t)p * N];
+    for (int p = 0; p < RW; ++p) w[p] = wp[(size_t)p * LS * N];
 #pragma unroll
-    for (int i = 0; i < DW; ++i) dl[i] = dp[(size_t)i * N];
+    for (int i = 0; i < DW; ++i) dl[i] = dp[(size_t)i * LS * N];
     if (comp)
         for (int c = 0; c < nl4; ++c) {
-            const float* s = Wsp + (size_t)(RW + 4 * c) * N + j;
-            Wl[(size_t)c * N + j] = float4{s[0], s[N], s[2 * (size_t)N], s[3 * (size_t)N]};
+            const float* s = wp + (size_t)(RW + 4 * c) * LS * N;
+            Wl[(size_t)c * LS * N + li] = float4{s[0], s[LS * (size_t)N], s[2 * LS * (size_t)N], s[3 * LS * (size_t)N]};
         }
 
     Lane<float> L;
@@ -1514,7 +1523,7 @@ __global__ __launch_bounds__(MAXT) void k_sim_sparse(const float* __restrict__ W
                 pre_ms = (ts + 1) % a.noise_period;
                 if (comp) nzn = noise2[(size_t)pre_ms * 10 * N + j];
             }
-            unsigned addr = sbase + (unsigned)(buf * ldS) * 4u;
+            unsigned addr = sbase + (unsigned)(buf * ldS + sl * KH) * 4u;
             float acc[4] = {0, 0, 0, 0};
             // eight slots: the eight addresses (byte b of the packed steps d0,
             // d1 added in turn), the eight gathers, then the eight products,
@@ -1547,25 +1556,26 @@ __global__ __launch_bounds__(MAXT) void k_sim_sparse(const float* __restrict__ W
 #pragma unroll
             for (int c = 0; c < LLMAX / 8; ++c)
                 if (c < nl8) {
-                    const float4 v0 = Wl[(size_t)(2 * c) * N + jc], v1 = Wl[(size_t)(2 * c + 1) * N + jc];
+                    const float4 v0 = Wl[(size_t)(2 * c) * LS * N + li], v1 = Wl[(size_t)(2 * c + 1) * LS * N + li];
                     const float wv[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
                     slot8(wv, dl[RW / 4 + 2 * c], dl[RW / 4 + 2 * c + 1]);
                 }
-            const float c = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+            float c = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+            if constexpr (LS == 2) c = lane_sum32<float>(c);   // the same sum in both lanes of a node
             // the dot is complete here, on every lane: left to itself the
             // compiler sinks the products into the node-lane branch below and
             // keeps every gathered value live until then
             asm volatile("" : : "v"(c));
             if (comp) {
                 Node<float>::step(c, gp, L.wEE, L.wEI, L.wIE, nz.x, nz.y, L.SE, L.SI, L.IE, L.II, L.rE, L.rI);
-                Sb[(buf ^ 1) * ldS + j] = L.SE;
+                if (sl == 0) Sb[(buf ^ 1) * ldS + j] = L.SE;
             }
             __syncthreads();
             buf ^= 1;
         }
         if (comp) L.bw.step(L.SE);
         if (ts % a.bold_tr == 0) {
-            if (comp) sample(a, sim, j, vol, L);
+            if (writer) sample(a, sim, j, vol, L);
             ++vol;
         }
         ++ts;
@@ -1575,7 +1585,10 @@ __global__ __launch_bounds__(MAXT) void k_sim_sparse(const float* __restrict__ W
                 const bool out = comp && fic_decide<float>(L.mIE, dur, trial, a.max_fic_trials, L.wIE, L.delta);
                 if (__syncthreads_or(out)) {
                     if (trial < a.max_fic_trials) {
-                        if (comp) { L.reset(); Sb[buf * ldS + j] = 0.001f; }
+                        if (comp) {
+                            L.reset();
+                            if (sl == 0) Sb[buf * ldS + j] = 0.001f;
+                        }
                         ts = 0; vol = 0; ++trial;
                         __syncthreads();
                     } else {
@@ -1587,7 +1600,7 @@ __global__ __launch_bounds__(MAXT) void k_sim_sparse(const float* __restrict__ W
             }
         }
     }
-    finish(a, sim, j, comp, vol, trial, failed, L);
+    finish(a, sim, j, writer, vol, trial, failed, L);
 }
 
 // ---------------------------------------------------------------------------
@@ -1864,82 +1877,111 @@ hipError_t launch_sim_hyb(const float* SCc, const SimArgs& a, int KS, size_t lds
     return hipGetLastError();
 }
 
-// Sparse row lists of an N x N row-major FP32 SC (see k_sim_sparse): returns
-// P (a multiple of 8; 0 for a matrix without nonzeros) and, when P fits the
-// kernel (P <= kSparseRW + kSparseLL), fills W[p * N + j] (weights) and
-// D[(p / 4) * N + j] (byte p % 4: the step of slot p, in bytes) for all
-// kSparseRW + kSparseLL slots, zero past P.
-int build_sparse_sc(const float* SC, int N, std::vector<float>& W, std::vector<unsigned>& D) {
+// Sparse row lists of an N x N row-major FP32 SC (see k_sim_sparse), each
+// row cut into LS slices of ceil(N / LS) nodes: returns P (a multiple of 8;
+// 0 for a matrix without nonzeros) and, when P fits the kernel (P <=
+// kSparseRW + kSparseLL), fills W[p * LS*N + sl*N + j] (weights) and
+// D[(p / 4) * LS*N + sl*N + j] (byte p % 4: the step of slot p, in bytes,
+// from the slice's first node) for all kSparseRW + kSparseLL slots, zero
+// past P.
+int build_sparse_sc(const float* SC, int N, int LS, std::vector<float>& W, std::vector<unsigned>& D) {
     constexpr size_t kSlots = kSparseRW + kSparseLL;
-    std::vector<std::vector<std::pair<float, unsigned>>> rows(N);
+    const int KH = (N + LS - 1) / LS;
+    const size_t M = (size_t)LS * N;   // lane columns: slice sl of row j at sl * N + j
+    std::vector<std::vector<std::pair<float, unsigned>>> rows(M);
     size_t P = 0;
-    for (int j = 0; j < N; ++j) {
-        unsigned at = 0;   // byte address of the previous slot's S_E
-        for (int k = 0; k < N; ++k) {
-            const float v = SC[(size_t)j * N + k];
-            if (v == 0.0f) continue;
-            const unsigned to = 4u * (unsigned)k;
-            while (to - at > kSparseMaxStep) {
-                rows[j].push_back({0.0f, kSparseMaxStep});
-                at += kSparseMaxStep;
+    for (int sl = 0; sl < LS; ++sl)
+        for (int j = 0; j < N; ++j) {
+            auto& row = rows[(size_t)sl * N + j];
+            unsigned at = 0;   // byte offset of the previous slot's S_E from the slice start
+            for (int k = sl * KH; k < std::min(N, (sl + 1) * KH); ++k) {
+                const float v = SC[(size_t)j * N + k];
+                if (v == 0.0f) continue;
+                const unsigned to = 4u * (unsigned)(k - sl * KH);
+                while (to - at > kSparseMaxStep) {
+                    row.push_back({0.0f, kSparseMaxStep});
+                    at += kSparseMaxStep;
+                }
+                row.push_back({v, to - at});
+                at = to;
             }
-            rows[j].push_back({v, to - at});
-            at = to;
+            P = std::max(P, row.size());
         }
-        P = std::max(P, rows[j].size());
-    }
     P = (P + 7) / 8 * 8;
     if (P > kSlots) return (int)P;   // too many for the kernel; nothing filled
-    W.assign(kSlots * N, 0.0f);
-    D.assign(kSlots / 4 * N, 0u);
-    for (int j = 0; j < N; ++j)
-        for (size_t p = 0; p < rows[j].size(); ++p) {
-            W[p * N + j] = rows[j][p].first;
-            D[p / 4 * N + j] |= rows[j][p].second << (8 * (p % 4));
+    W.assign(kSlots * M, 0.0f);
+    D.assign(kSlots / 4 * M, 0u);
+    // a row of n < P slots spreads its P - n padding slots (zero weight,
+    // step 0: a re-read of the current address) evenly, item i at the first
+    // slot p >= i * P / n, so that every lane walks k at the same pace: lanes
+    // of a wave then read nearby S_E, which sit in distinct LDS banks (or at
+    // one address), where padding at the end lets denser rows fall behind
+    // and collide in banks with the others
+    for (size_t j = 0; j < M; ++j) {
+        const size_t n = rows[j].size();
+        size_t i = 0;
+        for (size_t p = 0; p < P && i < n; ++p) {
+            if (p < i * P / n) continue;
+            W[p * M + j] = rows[j][i].first;
+            D[p / 4 * M + j] |= rows[j][i].second << (8 * (p % 4));
+            ++i;
         }
+    }
     return (int)P;
 }
 
 // register part: the slot count rounded up to 16 up to kSparseRW, beyond it
 // kSparseRW in registers and the rest (at most kSparseLL) in LDS
-SparseShape sparse_shape(int N, int P, size_t lds_bytes) {
+SparseShape sparse_shape(int N, int P, int LS, size_t lds_bytes) {
     SparseShape s{};
     s.P = P;
-    s.threads = ((N + 63) / 64) * 64;
+    s.LS = LS;
+    s.threads = ((N + 64 / LS - 1) / (64 / LS)) * 64;
     s.RW = P <= kSparseRW ? std::max(16, (P + 15) / 16 * 16) : kSparseRW;
     s.LL = P > kSparseRW ? P - kSparseRW : 0;
     const size_t ldS = (size_t)((N + 3) & ~3);
-    s.smem = (2 * ldS + (size_t)s.LL * N) * sizeof(float);
-    // (kStaticLds: the workgroup reductions' own LDS beside the dynamic part)
-    s.ok = N >= 1 && s.threads <= 512 && P > 0 && P % 8 == 0 && s.LL <= kSparseLL &&
-           s.smem + kStaticLds <= lds_bytes;
+    s.smem = (2 * ldS + (size_t)s.LL * LS * N) * sizeof(float);
+    // (kStaticLds: the workgroup reductions' own LDS beside the dynamic part;
+    // two lanes per node only without LDS-resident weights)
+    s.ok = N >= 1 && (LS == 1 || (LS == 2 && s.LL == 0)) && s.threads <= 512 && P > 0 && P % 8 == 0 &&
+           s.LL <= kSparseLL && s.smem + kStaticLds <= lds_bytes;
     return s;
 }
 
-template <int RW, int LLMAX>
+template <int RW, int LLMAX, int LS>
 static void launch_sparse(const float* W, const unsigned* D, const SimArgs& a, const SparseShape& s, hipStream_t st) {
-    hipLaunchKernelGGL((k_sim_sparse<RW, LLMAX, 512>), dim3(a.B), dim3(s.threads), s.smem, st, W, D, a, s.P);
+    hipLaunchKernelGGL((k_sim_sparse<RW, LLMAX, 512, LS>), dim3(a.B), dim3(s.threads), s.smem, st, W, D, a, s.P);
 }
 
-hipError_t launch_sim_sparse(const float* W, const unsigned* D, const SimArgs& a, int P, size_t lds_bytes,
-                             hipStream_t st) {
-    const SparseShape s = sparse_shape(a.N, P, lds_bytes);
-    if (!s.ok) return hipErrorInvalidValue;
+template <int LS>
+static hipError_t launch_sparse_ls(const float* W, const unsigned* D, const SimArgs& a, const SparseShape& s,
+                                   hipStream_t st) {
     static_assert(kSparseRW == 112 && kSparseLL == 96, "instantiations below");
     switch (s.RW) {
-        case 16: launch_sparse<16, 0>(W, D, a, s, st); break;
-        case 32: launch_sparse<32, 0>(W, D, a, s, st); break;
-        case 48: launch_sparse<48, 0>(W, D, a, s, st); break;
-        case 64: launch_sparse<64, 0>(W, D, a, s, st); break;
-        case 80: launch_sparse<80, 0>(W, D, a, s, st); break;
-        case 96: launch_sparse<96, 0>(W, D, a, s, st); break;
+        case 16: launch_sparse<16, 0, LS>(W, D, a, s, st); break;
+        case 32: launch_sparse<32, 0, LS>(W, D, a, s, st); break;
+        case 48: launch_sparse<48, 0, LS>(W, D, a, s, st); break;
+        case 64: launch_sparse<64, 0, LS>(W, D, a, s, st); break;
+        case 80: launch_sparse<80, 0, LS>(W, D, a, s, st); break;
+        case 96: launch_sparse<96, 0, LS>(W, D, a, s, st); break;
         case 112:
-            if (s.LL > 0) launch_sparse<112, 96>(W, D, a, s, st);
-            else launch_sparse<112, 0>(W, D, a, s, st);
+            if constexpr (LS == 1) {
+                if (s.LL > 0) launch_sparse<112, 96, 1>(W, D, a, s, st);
+                else launch_sparse<112, 0, 1>(W, D, a, s, st);
+            } else {
+                launch_sparse<112, 0, LS>(W, D, a, s, st);
+            }
             break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_sim_sparse(const float* W, const unsigned* D, const SimArgs& a, int P, int LS, size_t lds_bytes,
+                             hipStream_t st) {
+    const SparseShape s = sparse_shape(a.N, P, LS, lds_bytes);
+    if (!s.ok) return hipErrorInvalidValue;
+    return LS == 2 ? launch_sparse_ls<2>(W, D, a, s, st) : launch_sparse_ls<1>(W, D, a, s, st);
 }
 
 // FP64 hybrid shape: as hyb_shape with 8-byte values in 8-value chunks; the

@@ -3,8 +3,8 @@ N > 128 connectomes of partial density such as the synthetic 20-40 % SCs of
 the uneven-batch workload) against the CPU oracle, and its edge cases.
 
 Parity: N = 400 at 40 % density (about 200 slots per row: 112 register- and
-88 LDS-resident weights) and N = 200 at 33 % (88 slots, all in registers),
-FIC on, against the oracle's FP64 run: identical FIC trials, mean r_E within
+88 LDS-resident weights) and N = 200 at 33 % (88 slots, all in registers;
+and with two lanes per node, each taking half the row's k-range), FIC on, against the oracle's FP64 run: identical FIC trials, mean r_E within
 0.05 Hz, FC within 1e-3, FCD-KS within 0.01, per-TR BOLD within 1e-5
 relative (measured: 6.5e-7).  The sparse sum adds the same nonzero products
 as the dense one in another FP32 grouping, so against the dense hybrid
@@ -87,11 +87,13 @@ def _check_summary(oracle_mod, p, g, r, bold_rel=1e-5):
     assert rel <= bold_rel
 
 
-@pytest.mark.parametrize("N,seed,dens,want", [(400, 13, 0.40, "regs=112,lds=8"), (200, 12, 0.33, "lds=0")])
-def test_sparse_kernel_matches_oracle(oracle_mod, monkeypatch, N, seed, dens, want):
-    print(f"case: sparse N={N} density {dens}", flush=True)
+@pytest.mark.parametrize("N,seed,dens,ls,want", [(400, 13, 0.40, "1", "LS=1,P=200,regs=112,lds=88"),
+                                                  (200, 12, 0.33, "1", "LS=1,P=88,regs=96,lds=0"),
+                                                  (200, 12, 0.33, "2", "LS=2,")])
+def test_sparse_kernel_matches_oracle(oracle_mod, monkeypatch, N, seed, dens, ls, want):
+    print(f"case: sparse N={N} density {dens} LS={ls}", flush=True)
     p = _pop(N, W.synthetic_sc(N, dens, seed), 3, 10_000, seed)
-    g, name = _run(p, monkeypatch, {"BNM_SPARSE": "1"})
+    g, name = _run(p, monkeypatch, {"BNM_SPARSE": ls})
     print(name, flush=True)
     assert name.startswith("sparse_f32") and want in name
     _check_summary(oracle_mod, p, g, _oracle(oracle_mod, p))
