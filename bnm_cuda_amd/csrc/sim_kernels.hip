@@ -1441,9 +1441,9 @@ __global__ __launch_bounds__(MAXT) void k_sim_hyb(const float* __restrict__ SCc,
 // row): slot p holds the weight w_p and the step d_p = 4 * (k_p - k_{p-1})
 // in bytes (k_{-1} = 0) as one byte, four steps per 32-bit register.  A gap
 // of more than 63 nodes takes zero-weight slots of step 252; a row with
-// fewer nonzeros gets zero-weight slots of step 0 spread evenly between its
-// own (so that all lanes walk k at one pace and their gathers fall in
-// distinct LDS banks more often; build_sparse_sc).  The lane walks its row
+// fewer nonzeros gets zero-weight slots of step 0 between its own, every
+// item placed near slot k * P / N (build_sparse_sc), so that at each slot all
+// lanes gather S_E of nearby nodes, in distinct LDS banks.  The lane walks its row
 // with a running LDS byte address into the S_E buffer (one integer add per
 // slot), gathers S_E[k] with one 4-byte LDS read and multiplies it in.  The
 // first RW weights are register-resident, the remaining P - RW (at most
@@ -1888,7 +1888,8 @@ int build_sparse_sc(const float* SC, int N, int LS, std::vector<float>& W, std::
     constexpr size_t kSlots = kSparseRW + kSparseLL;
     const int KH = (N + LS - 1) / LS;
     const size_t M = (size_t)LS * N;   // lane columns: slice sl of row j at sl * N + j
-    std::vector<std::vector<std::pair<float, unsigned>>> rows(M);
+    struct Item { float w; unsigned step, at; };   // weight, step to it, its byte offset in the slice
+    std::vector<std::vector<Item>> rows(M);
     size_t P = 0;
     for (int sl = 0; sl < LS; ++sl)
         for (int j = 0; j < N; ++j) {
@@ -1899,10 +1900,10 @@ int build_sparse_sc(const float* SC, int N, int LS, std::vector<float>& W, std::
                 if (v == 0.0f) continue;
                 const unsigned to = 4u * (unsigned)(k - sl * KH);
                 while (to - at > kSparseMaxStep) {
-                    row.push_back({0.0f, kSparseMaxStep});
                     at += kSparseMaxStep;
+                    row.push_back({0.0f, kSparseMaxStep, at});
                 }
-                row.push_back({v, to - at});
+                row.push_back({v, to - at, to});
                 at = to;
             }
             P = std::max(P, row.size());
@@ -1911,20 +1912,24 @@ int build_sparse_sc(const float* SC, int N, int LS, std::vector<float>& W, std::
     if (P > kSlots) return (int)P;   // too many for the kernel; nothing filled
     W.assign(kSlots * M, 0.0f);
     D.assign(kSlots / 4 * M, 0u);
-    // a row of n < P slots spreads its P - n padding slots (zero weight,
-    // step 0: a re-read of the current address) evenly, item i at the first
-    // slot p >= i * P / n, so that every lane walks k at the same pace: lanes
-    // of a wave then read nearby S_E, which sit in distinct LDS banks (or at
-    // one address), where padding at the end lets denser rows fall behind
-    // and collide in banks with the others
+    // Slot placement by column: item i (node k_i of the slice) goes to slot
+    // k_i * P / KH, moved later where the previous item took that slot and
+    // earlier where the remaining items would not fit (at most P - n + i);
+    // slots between items hold zero weights with step 0 (re-reads of the
+    // current address).  All lanes then stand near the same node at every
+    // slot, within 32 consecutive S_E that lie in distinct LDS banks (or at
+    // one address), where items packed from slot 0 drift apart with the rows'
+    // densities and collide in banks.
     for (size_t j = 0; j < M; ++j) {
         const size_t n = rows[j].size();
-        size_t i = 0;
-        for (size_t p = 0; p < P && i < n; ++p) {
-            if (p < i * P / n) continue;
-            W[p * M + j] = rows[j][i].first;
-            D[p / 4 * M + j] |= rows[j][i].second << (8 * (p % 4));
-            ++i;
+        size_t prev = 0;
+        for (size_t i = 0; i < n; ++i) {
+            size_t p = (size_t)(rows[j][i].at / 4) * P / (size_t)KH;
+            if (i > 0) p = std::max(p, prev + 1);
+            p = std::min(p, P - n + i);
+            W[p * M + j] = rows[j][i].w;
+            D[p / 4 * M + j] |= rows[j][i].step << (8 * (p % 4));
+            prev = p;
         }
     }
     return (int)P;
