@@ -1200,6 +1200,7 @@ struct bnm_multi {
     double ms = 0;
     std::string plan;
     std::vector<int> sc_gen;   // each context's sc_gen at capture
+    std::vector<hipEvent_t> pair_events;   // fork / join of side-by-side stages
 };
 
 // Integration time estimate of a tuned context (ms): the tuner's time per
@@ -1226,6 +1227,39 @@ static int estimate_ms(bnm_ctx* x, double* out) {
 // workgroups of the context's integration kernel (one per simulation, or one
 // per 16 for the MFMA kernel)
 static int integration_wgs(const bnm_ctx* x) { return x->variant.kind == 2 ? (x->B + 15) / 16 : x->B; }
+
+// A "small" integration workgroup: at most 4 waves and 32 KB of LDS.  Two
+// such workgroups of different kernels always fit one CU together (8 waves,
+// two per SIMD at <= 256 VGPRs each), and a latency-bound small workgroup
+// runs about as fast beside another as alone (two 256-thread E/I workgroups
+// per CU take the time of one).
+static bool small_workgroup(const bnm_ctx* x) {
+    const Variant& v = x->variant;
+    const size_t el = x->f64 ? 8 : 4;
+    int threads = 1024;
+    size_t smem = 0;
+    switch (v.kind) {
+        case 0: { const FastShape s = fast_shape(x->N, v.ls, v.ks, el); threads = s.threads; smem = s.smem; break; }
+        case 1: { const FastShape s = stream_shape(x->N, v.ks, el); threads = s.threads; smem = s.smem; break; }
+        case 2: { const MfmaShape s = mfma_shape(x->N, v.mode); threads = s.threads; smem = s.smem; break; }
+        case 4: { const FastShape s = ei_shape(x->N, v.ls, v.mode == 1); threads = s.threads; smem = s.smem; break; }
+        case 5: {
+            const FastShape s = hyb_shape(x->N, v.ks, x->lds_max, x->hyb_rr, v.mode);
+            threads = s.threads; smem = s.smem; break;
+        }
+        case 6: { const FastShape s = ei64_shape(x->N, v.ls); threads = s.threads; smem = s.smem; break; }
+        case 7: {
+            const FastShape s = hyb64_shape(x->N, v.ks, x->lds_max, x->hyb_rr);
+            threads = s.threads; smem = s.smem; break;
+        }
+        case 8: {
+            const SparseShape s = sparse_shape(x->N, x->sp_P[v.ls - 1], v.ls, x->lds_max);
+            threads = s.threads; smem = s.smem; break;
+        }
+        default: threads = ((x->N + 63) / 64) * 64; smem = (size_t)x->N * el; break;
+    }
+    return threads <= 256 && smem <= 32 * 1024;
+}
 
 bnm_multi* bnm_multi_create(bnm_ctx** ctxs, int n, int* status) {
     auto set = [&](int s) { if (status) *status = s; };
@@ -1255,7 +1289,11 @@ bnm_multi* bnm_multi_create(bnm_ctx** ctxs, int n, int* status) {
     // the longest one down.  K = CUs / (largest workgroup count) branches run
     // concurrently; contexts go to branches longest first, each to the branch
     // with the least estimated work (LPT), and run one after another within
-    // it.  BNM_MULTI_BRANCHES overrides K (n: every context its own branch).
+    // it, except that two consecutive contexts with small workgroups
+    // (small_workgroup) run side by side on the CUs the branch holds (with
+    // 2K branches of CU-filling kernels that means: after the large context
+    // before them).  BNM_MULTI_BRANCHES overrides K (n: every context its own
+    // branch).
     int cus = 0, dev = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
@@ -1282,16 +1320,34 @@ bnm_multi* bnm_multi_create(bnm_ctx** ctxs, int n, int* status) {
         branch[b].push_back(i);
         load[b] += est[i];
     }
+    // stages of each branch: one context, or two small ones side by side
+    std::vector<std::vector<std::vector<int>>> stages(K);
+    for (int b = 0; b < K; ++b) {
+        const auto& br = branch[b];
+        for (size_t q = 0; q < br.size(); ++q) {
+            if (q + 1 < br.size() && small_workgroup(m->ctxs[br[q]]) && small_workgroup(m->ctxs[br[q + 1]])) {
+                stages[b].push_back({br[q], br[q + 1]});
+                ++q;
+            } else {
+                stages[b].push_back({br[q]});
+            }
+        }
+        load[b] = 0;
+        for (const auto& st : stages[b]) load[b] += st.size() == 2 ? std::max(est[st[0]], est[st[1]]) : est[st[0]];
+    }
     {
         char buf[96];
         for (int b = 0; b < K; ++b) {
             std::snprintf(buf, sizeof buf, "%sbranch %d (est %.0f ms):", b ? "; " : "", b, load[b]);
             m->plan += buf;
-            for (int i : branch[b]) {
-                std::snprintf(buf, sizeof buf, " ctx %d (N=%d, %d wg, est %.0f ms)", i, m->ctxs[i]->N,
-                              integration_wgs(m->ctxs[i]), est[i]);
-                m->plan += buf;
-            }
+            for (const auto& st : stages[b])
+                for (size_t q = 0; q < st.size(); ++q) {
+                    const int i = st[q];
+                    std::snprintf(buf, sizeof buf, "%s ctx %d (N=%d, %d wg, est %.0f ms)%s",
+                                  st.size() == 2 ? (q == 0 ? " [" : " |") : "", i, m->ctxs[i]->N,
+                                  integration_wgs(m->ctxs[i]), est[i], st.size() == 2 && q == 1 ? " ]" : "");
+                    m->plan += buf;
+                }
         }
     }
     // capture: the first context's stream forks to one stream per branch
@@ -1308,8 +1364,27 @@ bnm_multi* bnm_multi_create(bnm_ctx** ctxs, int n, int* status) {
     for (int b = 0; b < K && e == hipSuccess; ++b)
         if (bst[b] != s0) e = hipStreamWaitEvent(bst[b], m->fork, 0);
     for (int b = 0; b < K && e == hipSuccess && !r; ++b)
-        for (int i : branch[b])
-            if (!r) r = enqueue_run(m->ctxs[i], bst[b], false);
+        for (const auto& st : stages[b]) {
+            if (r || e != hipSuccess) break;
+            if (st.size() == 1) {
+                r = enqueue_run(m->ctxs[st[0]], bst[b], false);
+                continue;
+            }
+            // the second context on its own stream, forked from and joined to
+            // the branch's stream around the first
+            hipEvent_t f = nullptr, jn = nullptr;
+            e = hipEventCreateWithFlags(&f, hipEventDisableTiming);
+            if (e == hipSuccess) m->pair_events.push_back(f);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&jn, hipEventDisableTiming);
+            if (e == hipSuccess) m->pair_events.push_back(jn);
+            hipStream_t s2 = m->ctxs[st[1]]->st;
+            if (e == hipSuccess) e = hipEventRecord(f, bst[b]);
+            if (e == hipSuccess) e = hipStreamWaitEvent(s2, f, 0);
+            if (e == hipSuccess) r = enqueue_run(m->ctxs[st[0]], bst[b], false);
+            if (e == hipSuccess && !r) r = enqueue_run(m->ctxs[st[1]], s2, false);
+            if (e == hipSuccess && !r) e = hipEventRecord(jn, s2);
+            if (e == hipSuccess && !r) e = hipStreamWaitEvent(bst[b], jn, 0);
+        }
     for (int b = 0; b < K && e == hipSuccess && !r; ++b) {
         if (bst[b] == s0) continue;
         e = hipEventRecord(m->joins[b], bst[b]);
@@ -1352,6 +1427,8 @@ void bnm_multi_destroy(bnm_multi* m) {
     if (!m) return;
     if (m->exec) (void)hipGraphExecDestroy(m->exec);
     for (hipEvent_t e : {m->fork, m->t0, m->t1})
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : m->pair_events)
         if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : m->joins)
         if (e) (void)hipEventDestroy(e);

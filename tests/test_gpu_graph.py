@@ -117,8 +117,10 @@ def test_graph_replay_with_caller_fic_start_is_idempotent(precision):
 def test_uneven_populations_in_one_graph_launch(monkeypatch, branches):
     """Three populations with N = 20, 34, 68 and durations 60, 80, 70 s in one
     graph; each matches its own eager run bitwise.  Default packing puts each
-    in its own branch (their workgroups fit the CUs together); two branches
-    chain two populations in one; one branch runs all three in sequence."""
+    in its own branch (their workgroups fit the CUs together); with two
+    branches, the two populations that share one run side by side (small E/I
+    workgroups, a forked stage inside the branch); with one branch, two run
+    side by side and the third after them."""
     if branches is not None:
         monkeypatch.setenv("BNM_MULTI_BRANCHES", branches)
     specs = [(20, 6, 60_000, 3), (34, 5, 80_000, 4), (68, 17, 70_000, 5)]
@@ -135,6 +137,7 @@ def test_uneven_populations_in_one_graph_launch(monkeypatch, branches):
     plan = m.plan()
     print(plan)
     assert plan.count("branch") == (3 if branches is None else int(branches))
+    assert plan.count("[") == (0 if branches is None else 1)
     for rep in range(2):
         ms = m.run()
         assert ms > 0
