@@ -141,7 +141,7 @@ struct bnm_ctx {
     hipGraphExec_t graph_exec = nullptr;
     bool have_sc = false, have_noise = false, have_emp = false, have_params = false;
     int fast_ls = 0, fast_ks = 0, stream_ks = 0, force_stream = 0, no_ei = 0, ei_ls = 0, ei_bc = 0, ei_form = -1, no_hyb = 0, hyb_ks = 0, hyb_bc = -1, hyb_rr = -1;
-    int no_sparse = 0, sparse_only = 0;
+    int no_sparse = 0, sparse_only = 0, sparse_mode = 64;
     int sc_gen = 0;   // bumped when a new SC changes the sparse kernel's shape
     // no-allocation mode (bnm_run_simulations_noalloc): every device buffer is
     // carved out of `arena` (the library's static scratch array), the null
@@ -558,6 +558,9 @@ static bnm_ctx* ctx_create(const bnm_config* cfg, int* status, unsigned char* ar
     // of the tuner's candidates
     x->sparse_only = env_int("BNM_SPARSE", 0);
     x->no_sparse = env_int("BNM_NO_SPARSE", 0);
+    // BNM_SPARSE_STRICT=0: the sparse kernel with the plain FP32 node update
+    // instead of the strict one (the default)
+    x->sparse_mode = env_int("BNM_SPARSE_STRICT", 1) ? 64 : 0;
     x->hyb_ks = env_int("BNM_HYB_KS", 0);
     {
         int dev = 0, v = 0;
@@ -770,8 +773,8 @@ static std::string variant_name(const bnm_ctx* x, const Variant& v) {
                       v.mode == 2 ? ",dpp,pf3" : v.mode == 1 ? ",dpp" : "");
     } else if (v.kind == 8) {
         const SparseShape g = sparse_shape(x->N, x->sp_P[v.ls - 1], v.ls, x->lds_max);
-        std::snprintf(name, sizeof name, "sparse_f32(LS=%d,P=%d,regs=%d,lds=%d,threads=%d)", g.LS, g.P, g.RW, g.LL,
-                      g.threads);
+        std::snprintf(name, sizeof name, "sparse_f32(LS=%d,P=%d,regs=%d,lds=%d,threads=%d%s)", g.LS, g.P, g.RW, g.LL,
+                      g.threads, v.mode & 64 ? ",strict" : "");
     } else if (v.kind == 7) {
         const FastShape g = hyb64_shape(x->N, v.ks, x->lds_max, x->hyb_rr);
         std::snprintf(name, sizeof name, "hyb_f64(KS=%d,NW=%d,kc=%d,regs=%d,lds=%d,streamed=%d,threads=%d)", g.KS,
@@ -805,7 +808,9 @@ static hipError_t launch_variant(bnm_ctx* x, const Variant& v, const SimArgs& a,
         case 5: return launch_sim_hyb(x->SCc32, a, v.ks, x->lds_max, x->hyb_rr, v.mode, st);
         case 6: return launch_sim_ei64(x->SC64, a, v.ls, st);
         case 7: return launch_sim_hyb64(x->SCc64, a, v.ks, x->lds_max, x->hyb_rr, st);
-        case 8: return launch_sim_sparse(x->SPw[v.ls - 1], x->SPd[v.ls - 1], a, x->sp_P[v.ls - 1], v.ls, x->lds_max, st);
+        case 8:
+            return launch_sim_sparse(x->SPw[v.ls - 1], x->SPd[v.ls - 1], a, x->sp_P[v.ls - 1], v.ls, (v.mode & 64) != 0,
+                                     x->lds_max, st);
         default:
             return x->f64 ? launch_sim_plain<double>(x->SC64, a, st)
                           : launch_sim_plain<float>(x->SC32, a, st);
@@ -831,8 +836,8 @@ static std::vector<Variant> candidates(const bnm_ctx* x) {
         return out;
     }
     if (x->sparse_only && !x->f64) {
-        if (x->sparse_only & 1) add({8, 1, 1, 0});
-        if (x->sparse_only & 2) add({8, 2, 1, 0});
+        if (x->sparse_only & 1) add({8, 1, 1, x->sparse_mode});
+        if (x->sparse_only & 2) add({8, 2, 1, x->sparse_mode});
         if (!out.empty()) return out;
     }
     if (x->hyb_ks > 0) {
@@ -891,8 +896,8 @@ static std::vector<Variant> candidates(const bnm_ctx* x) {
         // kernels only, whose node update it shares (at N <= 128 the tuned
         // default is the strict E/I kernel)
         if (!x->f64 && !x->no_sparse && x->stream_ks <= 0) {
-            add({8, 1, 1, 0});
-            add({8, 2, 1, 0});
+            add({8, 1, 1, x->sparse_mode});
+            add({8, 2, 1, x->sparse_mode});
         }
     }
     // the MFMA kernel implements FP32 too; the tuner may pick it for large populations

@@ -57,8 +57,9 @@ struct SparseShape {
 };
 int build_sparse_sc(const float* SC, int N, int LS, std::vector<float>& W, std::vector<unsigned>& D);
 SparseShape sparse_shape(int N, int P, int LS, size_t lds_bytes);
-hipError_t launch_sim_sparse(const float* W, const unsigned* D, const SimArgs& a, int P, int LS, size_t lds_bytes,
-                             hipStream_t st);
+// strict: the strict FP32 node update (NodeStrict, FIC in FP64), else Node<float>
+hipError_t launch_sim_sparse(const float* W, const unsigned* D, const SimArgs& a, int P, int LS, bool strict,
+                             size_t lds_bytes, hipStream_t st);
 // FP32 split kernel with the node update divided between lanes by population
 FastShape ei_shape(int N, int LS, bool bc = false);
 // FP64 hybrid: register / LDS-resident SC, the rest streamed from the chunked

@@ -109,6 +109,87 @@ template <> struct Node<float> {
     }
 };
 
+// Strict FP32 node update, both pools in one lane: per pool the expressions of
+// the E/I kernel's strict short-chain form (k_sim_ei, FORM 3 | 64).  Every
+// coefficient that multiplies a state value is an FP32 hi + lo pair of its
+// FP64 value (w I_0, wEE / wEI, wIE, a G J_NMDA, dt gamma, dt / tau,
+// sigma sqrt_dt, -d log2 e), the exponent is taken from the rounded x, and
+// each S is an unevaluated sum S + Sl (the increment's rounding carried in
+// Sl by Fast2Sum, starting from the exact 0.001: kS0l).  What is left is the
+// FP32 rounding of x, exp2 and the reciprocal in the rate.
+struct NodeStrict {
+    float wEEh = 0, wEEl = 0, wEIh = 0, wEIl = 0, wIEh = 0, wIEl = 0;   // per node
+    float aGJh = 0, aGJl = 0, GJ = 0;                                    // per simulation
+    static constexpr float kS0l = (float)(0.001 - (double)0.001f);
+
+    static __device__ __forceinline__ void split(double v, float& h, float& l) {
+        h = (float)v;
+        l = (float)(v - (double)h);
+    }
+    // per simulation (the same in every lane of a workgroup)
+    __device__ __forceinline__ void set_G(double G) {
+        split(C64::a_E * (G * C64::J_NMDA), aGJh, aGJl);
+        GJ = (float)(G * C64::J_NMDA);
+    }
+    __device__ __forceinline__ void set_w(double wEE, double wEI, double wIE) {
+        split(wEE, wEEh, wEEl);
+        split(wEI, wEIh, wEIl);
+        split(wIE, wIEh, wIEl);
+    }
+    __device__ __forceinline__ void set_wIE(double wIE) { split(wIE, wIEh, wIEl); }
+
+    // one pool: x from pre / prel and the coupling, the rate, and S + Sl
+    // advanced by g r (1 - m S) - S dt / tau + noise
+    template <int P>
+    static __device__ __forceinline__ float pool(float pre, float prel, float c, float ch, float cl, float xi,
+                                                 float& S, float& Sl) {
+        constexpr double a64 = P == 0 ? C64::a_E : C64::a_I, b64 = P == 0 ? C64::b_E : C64::b_I;
+        constexpr double d64 = P == 0 ? C64::d_E : C64::d_I;
+        constexpr float aa = (float)a64, bb = (float)b64;
+        constexpr float ndlh = (float)(-d64 * kLog2e), ndll = (float)(-d64 * kLog2e - (double)ndlh);
+        constexpr float invd = (float)(1.0 / d64), d12 = (float)(d64 / 12.0);
+        constexpr double g64 = C64::dt * (P == 0 ? C64::gamma_E : C64::gamma_I);
+        constexpr double it64 = C64::dt / (P == 0 ? C64::tau_E : C64::tau_I);
+        constexpr float g = (float)g64, gl = (float)(g64 - (double)g);
+        constexpr float gm = P == 0 ? g : 0.0f, gml = P == 0 ? gl : 0.0f;
+        constexpr float it = (float)it64, itl = (float)(it64 - (double)it);
+        constexpr double nzs64 = C64::sigma * C64::sqrt_dt;
+        constexpr float nzs = (float)nzs64, nzsl = (float)(nzs64 - (double)nzs);
+        const float X0s = fmaf(aa, prel, fmaf(aa, pre, -bb));
+        float x = X0s;   // the inhibitory pool has no long-range coupling
+        if constexpr (P == 0) x = fmaf(ch, c, fmaf(cl, c, X0s));
+        const float y = fmaf(ndlh, x, ndll * x);
+        const float direct = x * __builtin_amdgcn_rcpf(1.0f - __builtin_amdgcn_exp2f(y));
+        const float series = invd + x * (0.5f + d12 * x);
+        const float r = fabsf(x) < 0.1f ? series : direct;
+        const float Ac = fmaf(-gm, S, g), Acl = fmaf(-gml, S, gl);
+        const float D = fmaf(xi, nzsl, fmaf(-itl, S, fmaf(-it, S, xi * nzs))) + Sl;
+        const float u = fmaf(r, Ac, fmaf(r, Acl, D));
+        const float t = S + u;
+        const float tc = __builtin_amdgcn_fmed3f(t, 0.0f, 1.0f);
+        Sl = tc == t ? u - (t - S) : 0.0f;
+        S = tc;
+        return r;
+    }
+
+    // one substep from the previous S values; I_E / I_I (hi + lo, rounded)
+    // only when asked (their uses are once per ms)
+    __device__ __forceinline__ void step(float c, float xiE, float xiI, float& SE, float& SEl, float& SI, float& SIl,
+                                         float& rE, float& rI, float* IE, float* II) const {
+        constexpr float kE0 = (float)C64::wE_I0, kE0l = (float)(C64::wE_I0 - (double)kE0);
+        constexpr float kI0 = (float)C64::wI_I0, kI0l = (float)(C64::wI_I0 - (double)kI0);
+        const float SEv = SE, SIv = SI;
+        const float preE = fmaf(-wIEh, SIv, fmaf(wEEh, SEv, kE0));
+        const float prelE = fmaf(-wIEl, SIv, fmaf(wEEl, SEv, kE0l));
+        const float preI = fmaf(-1.0f, SIv, fmaf(wEIh, SEv, kI0));
+        const float prelI = fmaf(wEIl, SEv, kI0l);
+        rE = pool<0>(preE, prelE, c, aGJh, aGJl, xiE, SE, SEl);
+        rI = pool<1>(preI, prelI, c, 0.0f, 0.0f, xiI, SI, SIl);
+        if (IE) *IE = fmaf(GJ, c, preE) + prelE;
+        if (II) *II = preI + prelI;
+    }
+};
+
 // Balloon-Windkessel state; one Euler step of 1 ms (bnm.cpp:205-211), update
 // order: x from old f, f from new x, nu and q from old f and (q) new nu.
 template <class R> struct BW;

@@ -1457,7 +1457,7 @@ __global__ __launch_bounds__(MAXT) void k_sim_hyb(const float* __restrict__ SCc,
 // Lanes past the last node walk node N - 1's row (the same addresses: LDS
 // broadcasts, no extra bank traffic) and keep nothing.
 // ---------------------------------------------------------------------------
-template <int RW, int LLMAX, int MAXT, int LS>
+template <int RW, int LLMAX, int MAXT, int LS, bool STRICT>
 __global__ __launch_bounds__(MAXT) void k_sim_sparse(const float* __restrict__ Wsp, const unsigned* __restrict__ Dsp,
                                                     SimArgs a, int P) {
     static_assert(RW % 8 == 0 && LLMAX % 8 == 0, "slots come in groups of eight");
@@ -1500,6 +1500,23 @@ __global__ __launch_bounds__(MAXT) void k_sim_sparse(const float* __restrict__ W
 
     Lane<float> L;
     if (comp) load_params(a, sim, j, L);
+    // strict update: coefficient hi / lo pairs, S lo parts, FIC in FP64
+    NodeStrict ns;
+    float SEl = NodeStrict::kS0l, SIl = NodeStrict::kS0l;
+    double wIEd = 0.0, deltad = a.init_delta;
+    // (the strict form sums its extended outputs in place in a.ext, once per
+    // TR sample, which keeps six FP64 sums out of the VGPRs)
+    double* extg = a.ext + (size_t)sim * 6 * N + j;
+    if constexpr (STRICT) {
+        ns.set_G(a.G[sim]);
+        if (comp) {
+            const size_t o = (size_t)sim * N + j;
+            wIEd = a.wIE_in[o];
+            ns.set_w(a.wEE[o], a.wEI[o], wIEd);
+        }
+        if (writer)
+            for (int i = 0; i < 6; ++i) extg[(size_t)i * N] = 0.0;
+    }
     const float gp = gain<float>(a.G[sim]);
     int ts = 0, vol = 0, trial = 0, buf = 0;
     bool adjusting = a.adjust_fic != 0, failed = false;
@@ -1567,7 +1584,10 @@ __global__ __launch_bounds__(MAXT) void k_sim_sparse(const float* __restrict__ W
             // keeps every gathered value live until then
             asm volatile("" : : "v"(c));
             if (comp) {
-                Node<float>::step(c, gp, L.wEE, L.wEI, L.wIE, nz.x, nz.y, L.SE, L.SI, L.IE, L.II, L.rE, L.rI);
+                if constexpr (STRICT)
+                    ns.step(c, nz.x, nz.y, L.SE, SEl, L.SI, SIl, L.rE, L.rI, &L.IE, &L.II);
+                else
+                    Node<float>::step(c, gp, L.wEE, L.wEI, L.wIE, nz.x, nz.y, L.SE, L.SI, L.IE, L.II, L.rE, L.rI);
                 if (sl == 0) Sb[(buf ^ 1) * ldS + j] = L.SE;
             }
             __syncthreads();
@@ -1575,20 +1595,46 @@ __global__ __launch_bounds__(MAXT) void k_sim_sparse(const float* __restrict__ W
         }
         if (comp) L.bw.step(L.SE);
         if (ts % a.bold_tr == 0) {
-            if (writer) sample(a, sim, j, vol, L);
+            if (writer) {
+                if constexpr (STRICT) {
+                    float* bold = reinterpret_cast<float*>(a.bold);
+                    bold[((size_t)sim * a.output_ts + vol) * N + j] = L.bw.bold(a.bw_k1, a.bw_k2, a.bw_k3);
+                    if (vol >= a.n_vols_remove) {
+                        extg[0] += (double)L.SE + (double)SEl;
+                        extg[(size_t)N] += (double)L.IE;
+                        extg[2 * (size_t)N] += (double)L.rE;
+                        extg[3 * (size_t)N] += (double)L.SI + (double)SIl;
+                        extg[4 * (size_t)N] += (double)L.II;
+                        extg[5 * (size_t)N] += (double)L.rI;
+                    }
+                } else {
+                    sample(a, sim, j, vol, L);
+                }
+            }
             ++vol;
         }
         ++ts;
         if (adjusting) {
             if (comp && ts >= a.i_start && ts <= a.i_end) L.mIE += (double)L.IE;
             if (ts == a.i_end) {
-                const bool out = comp && fic_decide<float>(L.mIE, dur, trial, a.max_fic_trials, L.wIE, L.delta);
+                bool out = false;
+                if (comp) {
+                    if constexpr (STRICT) {
+                        out = fic_decide<double>(L.mIE, dur, trial, a.max_fic_trials, wIEd, deltad);
+                        ns.set_wIE(wIEd);
+                    } else {
+                        out = fic_decide<float>(L.mIE, dur, trial, a.max_fic_trials, L.wIE, L.delta);
+                    }
+                }
                 if (__syncthreads_or(out)) {
                     if (trial < a.max_fic_trials) {
                         if (comp) {
                             L.reset();
+                            SEl = SIl = NodeStrict::kS0l;
                             if (sl == 0) Sb[buf * ldS + j] = 0.001f;
                         }
+                        if (STRICT && writer)
+                            for (int i = 0; i < 6; ++i) extg[(size_t)i * N] = 0.0;
                         ts = 0; vol = 0; ++trial;
                         __syncthreads();
                     } else {
@@ -1600,7 +1646,19 @@ __global__ __launch_bounds__(MAXT) void k_sim_sparse(const float* __restrict__ W
             }
         }
     }
-    finish(a, sim, j, writer, vol, trial, failed, L);
+    if constexpr (STRICT) {
+        if (writer) {
+            a.wIE_out[(size_t)sim * N + j] = wIEd;
+            const int n = vol - a.n_vols_remove;
+            for (int i = 0; i < 6; ++i) extg[(size_t)i * N] = n > 0 ? extg[(size_t)i * N] / n : 0.0;
+        }
+        if (threadIdx.x == 0) {
+            a.fic_failed[sim] = failed ? 1 : 0;
+            a.fic_trials[sim] = trial;
+        }
+    } else {
+        finish(a, sim, j, writer, vol, trial, failed, L);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1954,27 +2012,33 @@ SparseShape sparse_shape(int N, int P, int LS, size_t lds_bytes) {
 }
 
 template <int RW, int LLMAX, int LS>
-static void launch_sparse(const float* W, const unsigned* D, const SimArgs& a, const SparseShape& s, hipStream_t st) {
-    hipLaunchKernelGGL((k_sim_sparse<RW, LLMAX, 512, LS>), dim3(a.B), dim3(s.threads), s.smem, st, W, D, a, s.P);
+static void launch_sparse(const float* W, const unsigned* D, const SimArgs& a, const SparseShape& s, bool strict,
+                          hipStream_t st) {
+    if (strict)
+        hipLaunchKernelGGL((k_sim_sparse<RW, LLMAX, 512, LS, true>), dim3(a.B), dim3(s.threads), s.smem, st, W, D, a,
+                           s.P);
+    else
+        hipLaunchKernelGGL((k_sim_sparse<RW, LLMAX, 512, LS, false>), dim3(a.B), dim3(s.threads), s.smem, st, W, D, a,
+                           s.P);
 }
 
 template <int LS>
 static hipError_t launch_sparse_ls(const float* W, const unsigned* D, const SimArgs& a, const SparseShape& s,
-                                   hipStream_t st) {
+                                   bool strict, hipStream_t st) {
     static_assert(kSparseRW == 112 && kSparseLL == 96, "instantiations below");
     switch (s.RW) {
-        case 16: launch_sparse<16, 0, LS>(W, D, a, s, st); break;
-        case 32: launch_sparse<32, 0, LS>(W, D, a, s, st); break;
-        case 48: launch_sparse<48, 0, LS>(W, D, a, s, st); break;
-        case 64: launch_sparse<64, 0, LS>(W, D, a, s, st); break;
-        case 80: launch_sparse<80, 0, LS>(W, D, a, s, st); break;
-        case 96: launch_sparse<96, 0, LS>(W, D, a, s, st); break;
+        case 16: launch_sparse<16, 0, LS>(W, D, a, s, strict, st); break;
+        case 32: launch_sparse<32, 0, LS>(W, D, a, s, strict, st); break;
+        case 48: launch_sparse<48, 0, LS>(W, D, a, s, strict, st); break;
+        case 64: launch_sparse<64, 0, LS>(W, D, a, s, strict, st); break;
+        case 80: launch_sparse<80, 0, LS>(W, D, a, s, strict, st); break;
+        case 96: launch_sparse<96, 0, LS>(W, D, a, s, strict, st); break;
         case 112:
             if constexpr (LS == 1) {
-                if (s.LL > 0) launch_sparse<112, 96, 1>(W, D, a, s, st);
-                else launch_sparse<112, 0, 1>(W, D, a, s, st);
+                if (s.LL > 0) launch_sparse<112, 96, 1>(W, D, a, s, strict, st);
+                else launch_sparse<112, 0, 1>(W, D, a, s, strict, st);
             } else {
-                launch_sparse<112, 0, LS>(W, D, a, s, st);
+                launch_sparse<112, 0, LS>(W, D, a, s, strict, st);
             }
             break;
         default: return hipErrorInvalidValue;
@@ -1982,11 +2046,11 @@ static hipError_t launch_sparse_ls(const float* W, const unsigned* D, const SimA
     return hipGetLastError();
 }
 
-hipError_t launch_sim_sparse(const float* W, const unsigned* D, const SimArgs& a, int P, int LS, size_t lds_bytes,
-                             hipStream_t st) {
+hipError_t launch_sim_sparse(const float* W, const unsigned* D, const SimArgs& a, int P, int LS, bool strict,
+                             size_t lds_bytes, hipStream_t st) {
     const SparseShape s = sparse_shape(a.N, P, LS, lds_bytes);
     if (!s.ok) return hipErrorInvalidValue;
-    return LS == 2 ? launch_sparse_ls<2>(W, D, a, s, st) : launch_sparse_ls<1>(W, D, a, s, st);
+    return LS == 2 ? launch_sparse_ls<2>(W, D, a, s, strict, st) : launch_sparse_ls<1>(W, D, a, s, strict, st);
 }
 
 // FP64 hybrid shape: as hyb_shape with 8-byte values in 8-value chunks; the

@@ -10,6 +10,10 @@ relative (measured: 6.5e-7).  The sparse sum adds the same nonzero products
 as the dense one in another FP32 grouping, so against the dense hybrid
 kernel it agrees to FP32 rounding.
 
+The default sparse form has the strict FP32 node update (hi / lo
+coefficients, compensated S, FIC in FP64): per-TR BOLD within 1e-6 relative
+over the first 60 s at N = 200 and 400.
+
 Edge cases: a block-structured SC with isolated nodes (empty rows) and gaps
 of more than 63 nodes between a row's nonzeros (zero-weight step slots); an
 SC too dense for the kernel (it is not offered, and a pinned request falls
@@ -186,3 +190,34 @@ def test_sparse_kernel_graph_replays(monkeypatch):
         eager.run()
     gctx.close()
     eager.close()
+
+
+def _per_tr(g, r, nv=61):
+    rels = [np.abs(g["bold"][b][:nv] - r["bold"][b][:nv]).max() / np.abs(r["bold"][b][:nv]).max()
+            for b in range(g["bold"].shape[0])]
+    return max(rels), float(np.median(rels))
+
+
+def test_sparse_strict_per_tr_bold_within_1e6_over_60s(oracle_mod, monkeypatch):
+    """The default (strict) sparse form over the first 60 s: per-TR BOLD within
+    1e-6 relative.  N = 200 with two lanes per node against the oracle;
+    N = 400 against the FP64 hybrid kernel (itself within 1e-13 of the oracle
+    at N = 400 in the uneven-graph test), as the oracle takes minutes there."""
+    print("case: sparse strict N=200 LS=2 60 s vs oracle", flush=True)
+    p = _pop(200, W.synthetic_sc(200, 0.33, 12), 3, 60_000, 12)
+    g, name = _run(p, monkeypatch, {"BNM_SPARSE": "2"})
+    r = _oracle(oracle_mod, p)
+    mx, med = _per_tr(g, r)
+    print(name, f"| per-TR BOLD rel max {mx:.3g} median {med:.3g}", flush=True)
+    assert name.startswith("sparse_f32(LS=2") and name.endswith(",strict)")
+    np.testing.assert_array_equal(g["fic_trials"], r["fic_trials"])
+    assert mx <= 1e-6
+    print("case: sparse strict N=400 60 s vs FP64 kernel", flush=True)
+    p = _pop(400, W.synthetic_sc(400, 0.40, 13), 4, 60_000, 13)
+    g, name = _run(p, monkeypatch, {"BNM_SPARSE": "1"})
+    r, rname = _run(p, monkeypatch, {}, precision=bnm.PREC_F64)
+    mx, med = _per_tr(g, r)
+    print(name, "vs", rname, f"| per-TR BOLD rel max {mx:.3g} median {med:.3g}", flush=True)
+    assert name.endswith(",strict)")
+    np.testing.assert_array_equal(g["fic_trials"], r["fic_trials"])
+    assert mx <= 1e-6
