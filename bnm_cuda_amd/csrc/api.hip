@@ -101,6 +101,7 @@ struct bnm_ctx {
     // [LS * N]; sp_P the slots per lane of the current SC (0: no SC yet, or
     // too dense to fit)
     float* SPw[2] = {nullptr, nullptr};
+    double* SPw64[2] = {nullptr, nullptr};   // FP64 contexts: FP64 weights (8-byte steps)
     unsigned* SPd[2] = {nullptr, nullptr};
     int sp_P[2] = {0, 0};
     size_t lds_max = 64 * 1024;  // dynamic LDS a workgroup may use
@@ -330,7 +331,7 @@ int bnm_make_noise_chunked(unsigned seed, long long count, float* out, int threa
 
 void bnm_ctx_destroy(bnm_ctx* x) {
     if (!x) return;
-    void* bufs[] = {x->SC64, x->SC32, x->SCt64, x->SCt32, x->SCc32, x->SCc64, x->SPw[0], x->SPw[1], x->SPd[0], x->SPd[1], x->noise, x->G, x->wEE, x->wEI, x->wIE, x->wIEfic,
+    void* bufs[] = {x->SC64, x->SC32, x->SCt64, x->SCt32, x->SCc32, x->SCc64, x->SPw[0], x->SPw[1], x->SPw64[0], x->SPw64[1], x->SPd[0], x->SPd[1], x->noise, x->G, x->wEE, x->wEI, x->wIE, x->wIEfic,
                     x->wIEadj, x->wIEscratch, x->bold, x->ext, x->failed, x->trials, x->unstable,
                     x->seg_start, x->seg_len, x->pidx, x->mean, x->inv, x->wmean, x->winv, x->fc,
                     x->wfc, x->wz, x->wzlo, x->fcd, x->emp_c, x->ks_u, x->ks_fe, x->bins, x->nanf, x->corr, x->pen,
@@ -362,13 +363,16 @@ static int ctx_alloc(bnm_ctx* x) {
     if (x->f64) {
         if ((r = cdalloc(x, &x->SCc64, (N + kSCtPad) * N))) return r;
         HIPCHK(hipMemset(x->SCc64, 0, (N + kSCtPad) * N * sizeof(double)));
-    } else {
-        for (int i = 0; i < 2; ++i) {
-            const size_t ls = i + 1;
-            if (N > 512 / ls) continue;
+    }
+    for (int i = 0; i < 2; ++i) {
+        const size_t ls = i + 1;
+        if (N > 512 / ls) continue;
+        if (x->f64) {
+            if ((r = cdalloc(x, &x->SPw64[i], (size_t)(kSparseRW + kSparseLL) * ls * N))) return r;
+        } else {
             if ((r = cdalloc(x, &x->SPw[i], (size_t)(kSparseRW + kSparseLL) * ls * N))) return r;
-            if ((r = cdalloc(x, &x->SPd[i], (size_t)(kSparseRW + kSparseLL) / 4 * ls * N))) return r;
         }
+        if ((r = cdalloc(x, &x->SPd[i], (size_t)(kSparseRW + kSparseLL) / 4 * ls * N))) return r;
     }
     if ((r = cdalloc(x, &x->G, B))) return r;
     if ((r = cdalloc(x, &x->wEE, B * N))) return r;
@@ -622,15 +626,25 @@ int bnm_ctx_set_sc(bnm_ctx* x, const double* SC) {
     }
     int sp[2] = {0, 0};
     for (int i = 0; i < 2; ++i) {
-        if (!x->SPw[i]) continue;
-        std::vector<float> w;
+        if (!x->SPd[i]) continue;
         std::vector<unsigned> d;
-        int P = build_sparse_sc(f.data(), (int)N, i + 1, w, d);
-        if (P > kSparseRW + kSparseLL || P == 0) P = 0;   // too dense for the kernel (or empty)
-        else {
-            HIPCHK(hipMemcpy(x->SPw[i], w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(x->SPd[i], d.data(), d.size() * sizeof(unsigned), hipMemcpyHostToDevice));
+        int P = 0;
+        if (x->f64) {
+            std::vector<double> w;
+            P = build_sparse_sc(SC, (int)N, i + 1, w, d);
+            if (P > 0 && P <= kSparseRW64)
+                HIPCHK(hipMemcpy(x->SPw64[i], w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
+            else
+                P = 0;   // too dense for the FP64 kernel (or empty)
+        } else {
+            std::vector<float> w;
+            P = build_sparse_sc(f.data(), (int)N, i + 1, w, d);
+            if (P > 0 && P <= kSparseRW + kSparseLL)
+                HIPCHK(hipMemcpy(x->SPw[i], w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice));
+            else
+                P = 0;   // too dense for the kernel (or empty)
         }
+        if (P > 0) HIPCHK(hipMemcpy(x->SPd[i], d.data(), d.size() * sizeof(unsigned), hipMemcpyHostToDevice));
         sp[i] = P;
     }
     {
@@ -743,8 +757,8 @@ static bool variant_ok(const bnm_ctx* x, const Variant& v) {
         case 6: return x->f64 && ei64_shape(x->N, v.ls).ok;
         case 7: return x->f64 && hyb64_shape(x->N, v.ks, x->lds_max, x->hyb_rr).ok;
         case 8:
-            return !x->f64 && (v.ls == 1 || v.ls == 2) && x->sp_P[v.ls - 1] > 0 &&
-                   sparse_shape(x->N, x->sp_P[v.ls - 1], v.ls, x->lds_max).ok;
+            return (v.ls == 1 || v.ls == 2) && x->sp_P[v.ls - 1] > 0 && !(x->f64 && (v.mode & 64)) &&
+                   sparse_shape(x->N, x->sp_P[v.ls - 1], v.ls, x->lds_max, el).ok;
         default: return true;
     }
 }
@@ -772,9 +786,9 @@ static std::string variant_name(const bnm_ctx* x, const Variant& v) {
                       g.NW, g.kc, g.KC, g.SS, g.kc - g.KC - g.SS, g.threads,
                       v.mode == 2 ? ",dpp,pf3" : v.mode == 1 ? ",dpp" : "");
     } else if (v.kind == 8) {
-        const SparseShape g = sparse_shape(x->N, x->sp_P[v.ls - 1], v.ls, x->lds_max);
-        std::snprintf(name, sizeof name, "sparse_f32(LS=%d,P=%d,regs=%d,lds=%d,threads=%d%s)", g.LS, g.P, g.RW, g.LL,
-                      g.threads, v.mode & 64 ? ",strict" : "");
+        const SparseShape g = sparse_shape(x->N, x->sp_P[v.ls - 1], v.ls, x->lds_max, el);
+        std::snprintf(name, sizeof name, "sparse_%s(LS=%d,P=%d,regs=%d,lds=%d,threads=%d%s)", p, g.LS, g.P, g.RW,
+                      g.LL, g.threads, v.mode & 64 ? ",strict" : "");
     } else if (v.kind == 7) {
         const FastShape g = hyb64_shape(x->N, v.ks, x->lds_max, x->hyb_rr);
         std::snprintf(name, sizeof name, "hyb_f64(KS=%d,NW=%d,kc=%d,regs=%d,lds=%d,streamed=%d,threads=%d)", g.KS,
@@ -809,6 +823,9 @@ static hipError_t launch_variant(bnm_ctx* x, const Variant& v, const SimArgs& a,
         case 6: return launch_sim_ei64(x->SC64, a, v.ls, st);
         case 7: return launch_sim_hyb64(x->SCc64, a, v.ks, x->lds_max, x->hyb_rr, st);
         case 8:
+            if (x->f64)
+                return launch_sim_sparse64(x->SPw64[v.ls - 1], x->SPd[v.ls - 1], a, x->sp_P[v.ls - 1], v.ls, x->lds_max,
+                                           st);
             return launch_sim_sparse(x->SPw[v.ls - 1], x->SPd[v.ls - 1], a, x->sp_P[v.ls - 1], v.ls, (v.mode & 64) != 0,
                                      x->lds_max, st);
         default:
@@ -835,9 +852,10 @@ static std::vector<Variant> candidates(const bnm_ctx* x) {
         add({2, 0, 0, 0});
         return out;
     }
-    if (x->sparse_only && !x->f64) {
-        if (x->sparse_only & 1) add({8, 1, 1, x->sparse_mode});
-        if (x->sparse_only & 2) add({8, 2, 1, x->sparse_mode});
+    const int spm = x->f64 ? 0 : x->sparse_mode;   // FP64: the reference-order update
+    if (x->sparse_only) {
+        if (x->sparse_only & 1) add({8, 1, 1, spm});
+        if (x->sparse_only & 2) add({8, 2, 1, spm});
         if (!out.empty()) return out;
     }
     if (x->hyb_ks > 0) {
@@ -895,9 +913,9 @@ static std::vector<Variant> candidates(const bnm_ctx* x) {
         // sparse row lists (SCs of partial density); beside the streaming
         // kernels only, whose node update it shares (at N <= 128 the tuned
         // default is the strict E/I kernel)
-        if (!x->f64 && !x->no_sparse && x->stream_ks <= 0) {
-            add({8, 1, 1, x->sparse_mode});
-            add({8, 2, 1, x->sparse_mode});
+        if (!x->no_sparse && x->stream_ks <= 0) {
+            add({8, 1, 1, spm});
+            add({8, 2, 1, spm});
         }
     }
     // the MFMA kernel implements FP32 too; the tuner may pick it for large populations
@@ -1258,7 +1276,7 @@ static bool small_workgroup(const bnm_ctx* x) {
             threads = s.threads; smem = s.smem; break;
         }
         case 8: {
-            const SparseShape s = sparse_shape(x->N, x->sp_P[v.ls - 1], v.ls, x->lds_max);
+            const SparseShape s = sparse_shape(x->N, x->sp_P[v.ls - 1], v.ls, x->lds_max, el);
             threads = s.threads; smem = s.smem; break;
         }
         default: threads = ((x->N + 63) / 64) * 64; smem = (size_t)x->N * el; break;

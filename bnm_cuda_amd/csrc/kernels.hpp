@@ -43,8 +43,8 @@ hipError_t launch_sim_hyb(const float* SCc, const SimArgs& a, int KS, size_t lds
 // FP32 sparse kernel (N <= 512): each SC row as P slots of (weight, byte
 // step to the next nonzero's S_E), the first kSparseRW weights per lane in
 // registers and up to kSparseLL more in LDS; no SC traffic per substep
-constexpr unsigned kSparseMaxStep = 252;   // 63 nodes: the largest step one byte holds
 constexpr int kSparseRW = 112, kSparseLL = 96;
+constexpr int kSparseRW64 = 64;   // FP64: register-resident slots only
 constexpr size_t kStaticLds = 1024;   // headroom for a kernel's static LDS
 struct SparseShape {
     int P;         // slots per lane (multiple of 8)
@@ -55,11 +55,18 @@ struct SparseShape {
     size_t smem;
     bool ok;
 };
-int build_sparse_sc(const float* SC, int N, int LS, std::vector<float>& W, std::vector<unsigned>& D);
-SparseShape sparse_shape(int N, int P, int LS, size_t lds_bytes);
+// (T = float: 4-byte S_E, steps of up to 63 nodes; T = double: 8-byte S_E,
+// steps of up to 31 nodes)
+template <class T>
+int build_sparse_sc(const T* SC, int N, int LS, std::vector<T>& W, std::vector<unsigned>& D);
+SparseShape sparse_shape(int N, int P, int LS, size_t lds_bytes, size_t elem);
 // strict: the strict FP32 node update (NodeStrict, FIC in FP64), else Node<float>
 hipError_t launch_sim_sparse(const float* W, const unsigned* D, const SimArgs& a, int P, int LS, bool strict,
                              size_t lds_bytes, hipStream_t st);
+// FP64 sparse kernel: Node<double> in the reference's expression order, FP64
+// weights and S_E, all slots in registers (P <= kSparseRW64)
+hipError_t launch_sim_sparse64(const double* W, const unsigned* D, const SimArgs& a, int P, int LS, size_t lds_bytes,
+                               hipStream_t st);
 // FP32 split kernel with the node update divided between lanes by population
 FastShape ei_shape(int N, int LS, bool bc = false);
 // FP64 hybrid: register / LDS-resident SC, the rest streamed from the chunked

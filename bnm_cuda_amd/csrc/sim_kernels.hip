@@ -408,6 +408,10 @@ __device__ __forceinline__ unsigned lds_byte_addr(const void* p) {
 __device__ __forceinline__ float lds_f32(unsigned addr) {
     return *(const __attribute__((address_space(3))) float*)(size_t)addr;
 }
+template <class R>
+__device__ __forceinline__ R lds_load(unsigned addr) {
+    return *(const __attribute__((address_space(3))) R*)(size_t)addr;
+}
 
 // FP64 node update of one pool (the FP64 E/I kernel's step, below).
 __device__ __forceinline__ void ei_step_f64(double c, double Gl, double k0, double kSE, double kSI, double aa,
@@ -1457,10 +1461,13 @@ __global__ __launch_bounds__(MAXT) void k_sim_hyb(const float* __restrict__ SCc,
 // Lanes past the last node walk node N - 1's row (the same addresses: LDS
 // broadcasts, no extra bank traffic) and keep nothing.
 // ---------------------------------------------------------------------------
-template <int RW, int LLMAX, int MAXT, int LS, bool STRICT>
-__global__ __launch_bounds__(MAXT) void k_sim_sparse(const float* __restrict__ Wsp, const unsigned* __restrict__ Dsp,
+template <class R, int RW, int LLMAX, int MAXT, int LS, bool STRICT>
+__global__ __launch_bounds__(MAXT) void k_sim_sparse(const R* __restrict__ Wsp, const unsigned* __restrict__ Dsp,
                                                     SimArgs a, int P) {
     static_assert(RW % 8 == 0 && LLMAX % 8 == 0, "slots come in groups of eight");
+    static_assert(std::is_same<R, float>::value || (!STRICT && LLMAX == 0),
+                  "FP64: the reference-order update, all weights in registers");
+    constexpr unsigned ES = sizeof(R);
     static_assert(LS == 1 || LS == 2, "one or two lanes per node");
     constexpr int DW = (RW + LLMAX) / 4;
     constexpr int GN = 64 / LS;   // nodes per wave
@@ -1478,27 +1485,29 @@ __global__ __launch_bounds__(MAXT) void k_sim_sparse(const float* __restrict__ W
     const int nl4 = P > RW ? (P - RW) / 4 : 0;   // LDS-resident groups of four slots
     const int nl8 = nl4 / 2;                      // ... of eight (P - RW is a multiple of 8)
     extern __shared__ unsigned char smem_raw[];
-    float* Sb = reinterpret_cast<float*>(smem_raw);                      // [2][ldS]
-    float4* Wl = reinterpret_cast<float4*>(smem_raw + 8 * (size_t)ldS);  // [nl4][LS * N]
+    R* Sb = reinterpret_cast<R*>(smem_raw);                                   // [2][ldS]
+    float4* Wl = reinterpret_cast<float4*>(smem_raw + 2 * ES * (size_t)ldS);  // [nl4][LS * N] (FP32)
     const unsigned sbase = lds_byte_addr(Sb);
-    for (int i = threadIdx.x; i < 2 * ldS; i += blockDim.x) Sb[i] = (i % ldS) < N ? 0.001f : 0.0f;
+    for (int i = threadIdx.x; i < 2 * ldS; i += blockDim.x) Sb[i] = (i % ldS) < N ? R(0.001) : R(0);
     // Wsp / Dsp hold kSparseRW + kSparseLL slots (zero past P), so every
     // build reads them unguarded
-    float w[RW];
+    R w[RW];
     unsigned dl[DW];
-    const float* wp = Wsp + li;
+    const R* wp = Wsp + li;
     const unsigned* dp = Dsp + li;
 #pragma unroll
     for (int p = 0; p < RW; ++p) w[p] = wp[(size_t)p * LS * N];
 #pragma unroll
     for (int i = 0; i < DW; ++i) dl[i] = dp[(size_t)i * LS * N];
-    if (comp)
-        for (int c = 0; c < nl4; ++c) {
-            const float* s = wp + (size_t)(RW + 4 * c) * LS * N;
-            Wl[(size_t)c * LS * N + li] = float4{s[0], s[LS * (size_t)N], s[2 * LS * (size_t)N], s[3 * LS * (size_t)N]};
-        }
+    if constexpr (LLMAX > 0) {
+        if (comp)
+            for (int c = 0; c < nl4; ++c) {
+                const R* s = wp + (size_t)(RW + 4 * c) * LS * N;
+                Wl[(size_t)c * LS * N + li] = float4{s[0], s[LS * (size_t)N], s[2 * LS * (size_t)N], s[3 * LS * (size_t)N]};
+            }
+    }
 
-    Lane<float> L;
+    Lane<R> L;
     if (comp) load_params(a, sim, j, L);
     // strict update: coefficient hi / lo pairs, S lo parts, FIC in FP64
     NodeStrict ns;
@@ -1517,7 +1526,7 @@ __global__ __launch_bounds__(MAXT) void k_sim_sparse(const float* __restrict__ W
         if (writer)
             for (int i = 0; i < 6; ++i) extg[(size_t)i * N] = 0.0;
     }
-    const float gp = gain<float>(a.G[sim]);
+    const R gp = gain<R>(a.G[sim]);
     int ts = 0, vol = 0, trial = 0, buf = 0;
     bool adjusting = a.adjust_fic != 0, failed = false;
     const unsigned dur = (unsigned)(a.i_end - a.i_start + 1);
@@ -1540,14 +1549,14 @@ __global__ __launch_bounds__(MAXT) void k_sim_sparse(const float* __restrict__ W
                 pre_ms = (ts + 1) % a.noise_period;
                 if (comp) nzn = noise2[(size_t)pre_ms * 10 * N + j];
             }
-            unsigned addr = sbase + (unsigned)(buf * ldS + sl * KH) * 4u;
-            float acc[4] = {0, 0, 0, 0};
+            unsigned addr = sbase + (unsigned)(buf * ldS + sl * KH) * ES;
+            R acc[4] = {0, 0, 0, 0};
             // eight slots: the eight addresses (byte b of the packed steps d0,
             // d1 added in turn), the eight gathers, then the eight products,
             // so that eight LDS reads are in flight
             // (each batch its own scheduling region: hoisting later batches'
             // address chains costs registers the weights need)
-            auto slot8 = [&](const float (&wv)[8], unsigned d0, unsigned d1) {
+            auto slot8 = [&](const R (&wv)[8], unsigned d0, unsigned d1) {
                 __builtin_amdgcn_sched_barrier(0);
                 unsigned ad[8];
                 ad[0] = add_step<0>(addr, d0);
@@ -1559,26 +1568,29 @@ __global__ __launch_bounds__(MAXT) void k_sim_sparse(const float* __restrict__ W
                 ad[6] = add_step<2>(ad[5], d1);
                 ad[7] = add_step<3>(ad[6], d1);
                 addr = ad[7];
-                float sv[8];
+                R sv[8];
 #pragma unroll
-                for (int q = 0; q < 8; ++q) sv[q] = lds_f32(ad[q]);
+                for (int q = 0; q < 8; ++q) sv[q] = lds_load<R>(ad[q]);
 #pragma unroll
-                for (int q = 0; q < 8; ++q) acc[q & 3] = fmaf(wv[q], sv[q], acc[q & 3]);
+                for (int q = 0; q < 8; ++q) {
+                    if constexpr (std::is_same<R, float>::value) acc[q & 3] = fmaf(wv[q], sv[q], acc[q & 3]);
+                    else acc[q & 3] = fma(wv[q], sv[q], acc[q & 3]);
+                }
             };
 #pragma unroll
             for (int p = 0; p < RW; p += 8) {
-                const float wv[8] = {w[p], w[p + 1], w[p + 2], w[p + 3], w[p + 4], w[p + 5], w[p + 6], w[p + 7]};
+                const R wv[8] = {w[p], w[p + 1], w[p + 2], w[p + 3], w[p + 4], w[p + 5], w[p + 6], w[p + 7]};
                 slot8(wv, dl[p >> 2], dl[(p >> 2) + 1]);
             }
 #pragma unroll
             for (int c = 0; c < LLMAX / 8; ++c)
                 if (c < nl8) {
                     const float4 v0 = Wl[(size_t)(2 * c) * LS * N + li], v1 = Wl[(size_t)(2 * c + 1) * LS * N + li];
-                    const float wv[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+                    const R wv[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
                     slot8(wv, dl[RW / 4 + 2 * c], dl[RW / 4 + 2 * c + 1]);
                 }
-            float c = (acc[0] + acc[1]) + (acc[2] + acc[3]);
-            if constexpr (LS == 2) c = lane_sum32<float>(c);   // the same sum in both lanes of a node
+            R c = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+            if constexpr (LS == 2) c = lane_sum32<R>(c);   // the same sum in both lanes of a node
             // the dot is complete here, on every lane: left to itself the
             // compiler sinks the products into the node-lane branch below and
             // keeps every gathered value live until then
@@ -1587,7 +1599,7 @@ __global__ __launch_bounds__(MAXT) void k_sim_sparse(const float* __restrict__ W
                 if constexpr (STRICT)
                     ns.step(c, nz.x, nz.y, L.SE, SEl, L.SI, SIl, L.rE, L.rI, &L.IE, &L.II);
                 else
-                    Node<float>::step(c, gp, L.wEE, L.wEI, L.wIE, nz.x, nz.y, L.SE, L.SI, L.IE, L.II, L.rE, L.rI);
+                    Node<R>::step(c, gp, L.wEE, L.wEI, L.wIE, (R)nz.x, (R)nz.y, L.SE, L.SI, L.IE, L.II, L.rE, L.rI);
                 if (sl == 0) Sb[(buf ^ 1) * ldS + j] = L.SE;
             }
             __syncthreads();
@@ -1623,7 +1635,7 @@ __global__ __launch_bounds__(MAXT) void k_sim_sparse(const float* __restrict__ W
                         out = fic_decide<double>(L.mIE, dur, trial, a.max_fic_trials, wIEd, deltad);
                         ns.set_wIE(wIEd);
                     } else {
-                        out = fic_decide<float>(L.mIE, dur, trial, a.max_fic_trials, L.wIE, L.delta);
+                        out = fic_decide<R>(L.mIE, dur, trial, a.max_fic_trials, L.wIE, L.delta);
                     }
                 }
                 if (__syncthreads_or(out)) {
@@ -1631,7 +1643,7 @@ __global__ __launch_bounds__(MAXT) void k_sim_sparse(const float* __restrict__ W
                         if (comp) {
                             L.reset();
                             SEl = SIl = NodeStrict::kS0l;
-                            if (sl == 0) Sb[buf * ldS + j] = 0.001f;
+                            if (sl == 0) Sb[buf * ldS + j] = R(0.001);
                         }
                         if (STRICT && writer)
                             for (int i = 0; i < 6; ++i) extg[(size_t)i * N] = 0.0;
@@ -1942,11 +1954,13 @@ hipError_t launch_sim_hyb(const float* SCc, const SimArgs& a, int KS, size_t lds
 // D[(p / 4) * LS*N + sl*N + j] (byte p % 4: the step of slot p, in bytes,
 // from the slice's first node) for all kSparseRW + kSparseLL slots, zero
 // past P.
-int build_sparse_sc(const float* SC, int N, int LS, std::vector<float>& W, std::vector<unsigned>& D) {
+template <class T>
+int build_sparse_sc(const T* SC, int N, int LS, std::vector<T>& W, std::vector<unsigned>& D) {
     constexpr size_t kSlots = kSparseRW + kSparseLL;
+    constexpr unsigned ES = sizeof(T), kMaxStep = 255 / ES * ES;   // the largest step one byte holds
     const int KH = (N + LS - 1) / LS;
     const size_t M = (size_t)LS * N;   // lane columns: slice sl of row j at sl * N + j
-    struct Item { float w; unsigned step, at; };   // weight, step to it, its byte offset in the slice
+    struct Item { T w; unsigned step, at; };   // weight, step to it, its byte offset in the slice
     std::vector<std::vector<Item>> rows(M);
     size_t P = 0;
     for (int sl = 0; sl < LS; ++sl)
@@ -1954,12 +1968,12 @@ int build_sparse_sc(const float* SC, int N, int LS, std::vector<float>& W, std::
             auto& row = rows[(size_t)sl * N + j];
             unsigned at = 0;   // byte offset of the previous slot's S_E from the slice start
             for (int k = sl * KH; k < std::min(N, (sl + 1) * KH); ++k) {
-                const float v = SC[(size_t)j * N + k];
-                if (v == 0.0f) continue;
-                const unsigned to = 4u * (unsigned)(k - sl * KH);
-                while (to - at > kSparseMaxStep) {
-                    at += kSparseMaxStep;
-                    row.push_back({0.0f, kSparseMaxStep, at});
+                const T v = SC[(size_t)j * N + k];
+                if (v == T(0)) continue;
+                const unsigned to = ES * (unsigned)(k - sl * KH);
+                while (to - at > kMaxStep) {
+                    at += kMaxStep;
+                    row.push_back({T(0), kMaxStep, at});
                 }
                 row.push_back({v, to - at, to});
                 at = to;
@@ -1968,7 +1982,7 @@ int build_sparse_sc(const float* SC, int N, int LS, std::vector<float>& W, std::
         }
     P = (P + 7) / 8 * 8;
     if (P > kSlots) return (int)P;   // too many for the kernel; nothing filled
-    W.assign(kSlots * M, 0.0f);
+    W.assign(kSlots * M, T(0));
     D.assign(kSlots / 4 * M, 0u);
     // Slot placement by column: item i (node k_i of the slice) goes to slot
     // k_i * P / KH, moved later where the previous item took that slot and
@@ -1982,7 +1996,7 @@ int build_sparse_sc(const float* SC, int N, int LS, std::vector<float>& W, std::
         const size_t n = rows[j].size();
         size_t prev = 0;
         for (size_t i = 0; i < n; ++i) {
-            size_t p = (size_t)(rows[j][i].at / 4) * P / (size_t)KH;
+            size_t p = (size_t)(rows[j][i].at / ES) * P / (size_t)KH;
             if (i > 0) p = std::max(p, prev + 1);
             p = std::min(p, P - n + i);
             W[p * M + j] = rows[j][i].w;
@@ -1993,9 +2007,13 @@ int build_sparse_sc(const float* SC, int N, int LS, std::vector<float>& W, std::
     return (int)P;
 }
 
+template int build_sparse_sc<float>(const float*, int, int, std::vector<float>&, std::vector<unsigned>&);
+template int build_sparse_sc<double>(const double*, int, int, std::vector<double>&, std::vector<unsigned>&);
+
 // register part: the slot count rounded up to 16 up to kSparseRW, beyond it
-// kSparseRW in registers and the rest (at most kSparseLL) in LDS
-SparseShape sparse_shape(int N, int P, int LS, size_t lds_bytes) {
+// kSparseRW in registers and the rest (at most kSparseLL) in LDS; FP64
+// (elem 8): at most kSparseRW64 slots, all in registers
+SparseShape sparse_shape(int N, int P, int LS, size_t lds_bytes, size_t elem) {
     SparseShape s{};
     s.P = P;
     s.LS = LS;
@@ -2003,7 +2021,11 @@ SparseShape sparse_shape(int N, int P, int LS, size_t lds_bytes) {
     s.RW = P <= kSparseRW ? std::max(16, (P + 15) / 16 * 16) : kSparseRW;
     s.LL = P > kSparseRW ? P - kSparseRW : 0;
     const size_t ldS = (size_t)((N + 3) & ~3);
-    s.smem = (2 * ldS + (size_t)s.LL * LS * N) * sizeof(float);
+    s.smem = 2 * ldS * elem + (size_t)s.LL * LS * N * sizeof(float);
+    if (elem == 8 && (s.LL > 0 || s.RW > kSparseRW64)) {
+        s.ok = false;
+        return s;
+    }
     // (kStaticLds: the workgroup reductions' own LDS beside the dynamic part;
     // two lanes per node only without LDS-resident weights)
     s.ok = N >= 1 && (LS == 1 || (LS == 2 && s.LL == 0)) && s.threads <= 512 && P > 0 && P % 8 == 0 &&
@@ -2015,11 +2037,34 @@ template <int RW, int LLMAX, int LS>
 static void launch_sparse(const float* W, const unsigned* D, const SimArgs& a, const SparseShape& s, bool strict,
                           hipStream_t st) {
     if (strict)
-        hipLaunchKernelGGL((k_sim_sparse<RW, LLMAX, 512, LS, true>), dim3(a.B), dim3(s.threads), s.smem, st, W, D, a,
-                           s.P);
+        hipLaunchKernelGGL((k_sim_sparse<float, RW, LLMAX, 512, LS, true>), dim3(a.B), dim3(s.threads), s.smem, st, W,
+                           D, a, s.P);
     else
-        hipLaunchKernelGGL((k_sim_sparse<RW, LLMAX, 512, LS, false>), dim3(a.B), dim3(s.threads), s.smem, st, W, D, a,
-                           s.P);
+        hipLaunchKernelGGL((k_sim_sparse<float, RW, LLMAX, 512, LS, false>), dim3(a.B), dim3(s.threads), s.smem, st,
+                           W, D, a, s.P);
+}
+
+template <int RW, int LS>
+static void launch_sparse64(const double* W, const unsigned* D, const SimArgs& a, const SparseShape& s,
+                            hipStream_t st) {
+    hipLaunchKernelGGL((k_sim_sparse<double, RW, 0, 512, LS, false>), dim3(a.B), dim3(s.threads), s.smem, st, W, D,
+                       a, s.P);
+}
+
+hipError_t launch_sim_sparse64(const double* W, const unsigned* D, const SimArgs& a, int P, int LS, size_t lds_bytes,
+                               hipStream_t st) {
+    const SparseShape s = sparse_shape(a.N, P, LS, lds_bytes, sizeof(double));
+    if (!s.ok) return hipErrorInvalidValue;
+    static_assert(kSparseRW64 == 64, "instantiations below");
+    const bool two = LS == 2;
+    switch (s.RW) {
+        case 16: two ? launch_sparse64<16, 2>(W, D, a, s, st) : launch_sparse64<16, 1>(W, D, a, s, st); break;
+        case 32: two ? launch_sparse64<32, 2>(W, D, a, s, st) : launch_sparse64<32, 1>(W, D, a, s, st); break;
+        case 48: two ? launch_sparse64<48, 2>(W, D, a, s, st) : launch_sparse64<48, 1>(W, D, a, s, st); break;
+        case 64: two ? launch_sparse64<64, 2>(W, D, a, s, st) : launch_sparse64<64, 1>(W, D, a, s, st); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 template <int LS>
@@ -2048,7 +2093,7 @@ static hipError_t launch_sparse_ls(const float* W, const unsigned* D, const SimA
 
 hipError_t launch_sim_sparse(const float* W, const unsigned* D, const SimArgs& a, int P, int LS, bool strict,
                              size_t lds_bytes, hipStream_t st) {
-    const SparseShape s = sparse_shape(a.N, P, LS, lds_bytes);
+    const SparseShape s = sparse_shape(a.N, P, LS, lds_bytes, sizeof(float));
     if (!s.ok) return hipErrorInvalidValue;
     return LS == 2 ? launch_sparse_ls<2>(W, D, a, s, strict, st) : launch_sparse_ls<1>(W, D, a, s, strict, st);
 }

@@ -14,6 +14,9 @@ The default sparse form has the strict FP32 node update (hi / lo
 coefficients, compensated S, FIC in FP64): per-TR BOLD within 1e-6 relative
 over the first 60 s at N = 200 and 400.
 
+The FP64 form (N <= 256 with two lanes per node, all slots in registers)
+meets the FP64 bounds against the oracle.
+
 Edge cases: a block-structured SC with isolated nodes (empty rows) and gaps
 of more than 63 nodes between a row's nonzeros (zero-weight step slots); an
 SC too dense for the kernel (it is not offered, and a pinned request falls
@@ -221,3 +224,23 @@ def test_sparse_strict_per_tr_bold_within_1e6_over_60s(oracle_mod, monkeypatch):
     assert name.endswith(",strict)")
     np.testing.assert_array_equal(g["fic_trials"], r["fic_trials"])
     assert mx <= 1e-6
+
+
+def test_sparse_f64_matches_oracle(oracle_mod, monkeypatch):
+    """FP64 sparse kernel (two lanes per node, FP64 weights and S_E, the
+    reference-order node update): against the oracle at the FP64 bounds of
+    the uneven-graph test (per-TR BOLD 1e-6 relative, cost and FC 1e-9),
+    same FIC trials."""
+    print("case: sparse f64 N=200 LS=2", flush=True)
+    p = _pop(200, W.synthetic_sc(200, 0.33, 12), 3, 10_000, 12)
+    g, name = _run(p, monkeypatch, {"BNM_SPARSE": "2"}, precision=bnm.PREC_F64)
+    r = _oracle(oracle_mod, p)
+    rel = max(np.abs(g["bold"][b] - r["bold"][b]).max() / np.abs(r["bold"][b]).max() for b in range(p["B"]))
+    print(name, f"| per-TR BOLD rel {rel:.3g}, cost {np.abs(g['corr'] - r['corr']).max():.3g}, "
+          f"FC {np.abs(g['fc'] - r['fc']).max():.3g}", flush=True)
+    assert name.startswith("sparse_f64(LS=2")
+    np.testing.assert_array_equal(g["fic_trials"], r["fic_trials"])
+    assert rel <= 1e-6
+    np.testing.assert_allclose(g["corr"], r["corr"], rtol=0, atol=1e-9)
+    np.testing.assert_allclose(g["fc"], r["fc"], rtol=0, atol=1e-9)
+    np.testing.assert_allclose(g["ext"], r["ext"], rtol=1e-7, atol=1e-9)
