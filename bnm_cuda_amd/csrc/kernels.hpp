@@ -40,9 +40,10 @@ FastShape hyb_shape(int N, int KS, size_t lds_bytes, int rr = -1, int form = 0);
 // broadcast with three streamed chunks in flight
 hipError_t launch_sim_hyb(const float* SCc, const SimArgs& a, int KS, size_t lds_bytes, int rr, int form,
                           hipStream_t st);
-// FP32 sparse kernel (N <= 512): each SC row as P slots of (weight, byte
-// step to the next nonzero's S_E), the first kSparseRW weights per lane in
-// registers and up to kSparseLL more in LDS; no SC traffic per substep
+// Sparse kernel (N <= 512; FP32 and FP64): each SC row (or half-row, two
+// lanes per node) as P slots of (weight, byte step to the next nonzero's
+// S_E), the first kSparseRW weights per lane in registers and in FP32 up to
+// kSparseLL more in LDS; no SC traffic per substep
 constexpr int kSparseRW = 112, kSparseLL = 96;
 constexpr int kSparseRW64 = 64;   // FP64: register-resident slots only
 constexpr size_t kStaticLds = 1024;   // headroom for a kernel's static LDS

@@ -1438,27 +1438,36 @@ __global__ __launch_bounds__(MAXT) void k_sim_hyb(const float* __restrict__ SCc,
 
 // ---------------------------------------------------------------------------
 // sparse (N <= 512): the SC rows as lists of their nonzeros, for connectomes
-// of partial density (the synthetic 20-40 % SCs).  One lane per node, one
-// workgroup of ceil(N / 64) waves per simulation, one barrier per substep.
+// of partial density (the synthetic 20-40 % SCs).  LS = 1 or 2 lanes per node
+// (LS = 2: each lane takes half of the row's k-range and the two partial dots
+// are summed with a permlane32 swap), one workgroup of ceil(N * LS / 64)
+// waves per simulation, one barrier per substep.  R = float or double (the
+// element type of the weights and of S_E in LDS; ES = sizeof(R) bytes).
 //
-// Row j's nonzeros in increasing k become P slots (P the same for every
-// row): slot p holds the weight w_p and the step d_p = 4 * (k_p - k_{p-1})
-// in bytes (k_{-1} = 0) as one byte, four steps per 32-bit register.  A gap
-// of more than 63 nodes takes zero-weight slots of step 252; a row with
-// fewer nonzeros gets zero-weight slots of step 0 between its own, every
-// item placed near slot k * P / N (build_sparse_sc), so that at each slot all
-// lanes gather S_E of nearby nodes, in distinct LDS banks.  The lane walks its row
-// with a running LDS byte address into the S_E buffer (one integer add per
-// slot), gathers S_E[k] with one 4-byte LDS read and multiplies it in.  The
-// first RW weights are register-resident, the remaining P - RW (at most
-// LLMAX) sit in LDS as [slot / 4][N] float4; every step is in registers.
-// Nothing is streamed: at N = 400 and 40 % density (P ~ 200) the whole
+// The nonzeros of a lane's range, in increasing k, become P slots (P the
+// same for every lane): slot p holds the weight w_p and the step
+// d_p = ES * (k_p - k_{p-1}) in bytes (from the range's first node) as one
+// byte, four steps per 32-bit register.  A gap of more than 255 / ES nodes
+// takes zero-weight slots of the largest step; a lane with fewer nonzeros
+// gets zero-weight slots of step 0 between its own, every item placed near
+// slot k * P / (range width) (build_sparse_sc), so that at each slot all
+// lanes gather S_E of nearby nodes, in distinct LDS banks.  The lane walks
+// its slots with a running LDS byte address into the S_E buffer (one
+// integer add per slot, the byte picked by an SDWA select), gathers S_E with
+// one LDS read and multiplies it in, eight slots per batch.  The first RW
+// weights are register-resident; in FP32 the remaining P - RW (at most LLMAX)
+// sit in LDS as [slot / 4][LS * N] float4; every step is in registers.
+// Nothing is streamed: at N = 400 and 40 % density (P = 200) the whole
 // matrix is on the CU, where the dense FP32 matrix (640 KB) does not fit.
 //
+// STRICT (FP32): the node update is NodeStrict (hi / lo coefficients, S as
+// S + Sl, FIC in FP64, extended-output sums in place in a.ext); otherwise
+// Node<R> (FP64: the reference's expression order).
+//
 // A zero weight adds +0 to the dense dot as well (S_E is finite and >= 0),
-// so this kernel and the dense ones differ only in the FP32 grouping of the
-// sum: four partial sums by slot index here, by k in the dense kernels.
-// Lanes past the last node walk node N - 1's row (the same addresses: LDS
+// so this kernel and the dense ones differ only in the grouping of the sum:
+// four partial sums by slot index here, by k in the dense kernels.  Lanes
+// past the last node walk node N - 1's slots (the same addresses: LDS
 // broadcasts, no extra bank traffic) and keep nothing.
 // ---------------------------------------------------------------------------
 template <class R, int RW, int LLMAX, int MAXT, int LS, bool STRICT>
