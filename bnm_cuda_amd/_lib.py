@@ -28,7 +28,7 @@ FC_F64, FC_F32, FC_BF16, FC_BF16X3 = 0, 1, 2, 3
 # every symbol include/bnm_hip.h declares
 EXPORTS = (
     "bnm_default_config", "bnm_finalize_config", "bnm_output_ts", "bnm_n_pairs", "bnm_n_windows",
-    "bnm_window_bounds", "bnm_regional_params", "bnm_analytical_fic", "bnm_make_noise", "bnm_make_noise_chunked", "bnm_run_simulations", "bnm_run_simulations_noalloc", "bnm_ctx_create",
+    "bnm_window_bounds", "bnm_regional_params", "bnm_analytical_fic", "bnm_make_noise", "bnm_make_noise_chunked", "bnm_sparse_layout", "bnm_run_simulations", "bnm_run_simulations_noalloc", "bnm_ctx_create",
     "bnm_ctx_destroy", "bnm_ctx_set_sc", "bnm_ctx_set_noise", "bnm_ctx_set_empirical",
     "bnm_ctx_set_params", "bnm_ctx_run", "bnm_ctx_set_bold", "bnm_ctx_analyze", "bnm_ctx_get_hemi_fcd", "bnm_ctx_get_outputs", "bnm_ctx_timings",
     "bnm_ctx_kernel_name", "bnm_ctx_tune_log", "bnm_multi_create", "bnm_multi_run",
@@ -88,6 +88,7 @@ def lib():
     L.bnm_regional_params.argtypes = [C.c_int, C.c_int, pd, C.c_double, pd, pd]
     L.bnm_analytical_fic.argtypes = [C.c_int, C.c_int, pd, pd, pd, pd, pd, pi]
     L.bnm_make_noise_chunked.argtypes = [C.c_uint, C.c_longlong, pf, C.c_int]
+    L.bnm_sparse_layout.argtypes = [C.c_int, pd, C.c_int, C.c_int, pi, pd, C.POINTER(C.c_uint)]
     L.bnm_run_simulations.argtypes = [pc, pd, pd, pd, pd, pd, pd, pd, pd, pf, C.c_longlong,
                                       pd, C.c_int, pd, C.c_int] + [pd] * 7 + [pi, pi, pd, pd]
     L.bnm_run_simulations_noalloc.argtypes = L.bnm_run_simulations.argtypes
@@ -221,6 +222,24 @@ def make_noise(seed: int, count: int) -> np.ndarray:
     out = np.empty(count, np.float32)
     _chk(lib().bnm_make_noise(seed, count, _p(out, C.c_float)))
     return out
+
+
+def sparse_layout(SC, lanes_per_node: int = 1, fp64: bool = False):
+    """The sparse kernel's nonzero lists for SC (host only): (P, W [P, LS*N]
+    float64, D [P/4, LS*N] uint32); P = 0 (and W, D None) where the kernel
+    does not apply."""
+    SC = np.ascontiguousarray(SC, np.float64)
+    N = SC.shape[0]
+    P = C.c_int(0)
+    _chk(lib().bnm_sparse_layout(N, _p(SC, C.c_double), lanes_per_node, int(fp64), C.byref(P), None, None))
+    if P.value == 0:
+        return 0, None, None
+    M = lanes_per_node * N
+    W = np.empty((P.value, M), np.float64)
+    D = np.empty((P.value // 4, M), np.uint32)
+    _chk(lib().bnm_sparse_layout(N, _p(SC, C.c_double), lanes_per_node, int(fp64), C.byref(P),
+                                 _p(W, C.c_double), D.ctypes.data_as(C.POINTER(C.c_uint))))
+    return P.value, W, D
 
 
 def make_noise_chunked(seed: int, count: int, threads: int = 0) -> np.ndarray:

@@ -306,6 +306,33 @@ int bnm_make_noise(unsigned seed, long long count, float* out) {
     return BNM_OK;
 }
 
+int bnm_sparse_layout(int nodes, const double* SC, int lanes_per_node, int fp64, int* slots, double* W,
+                      unsigned* D) {
+    if (nodes < 1 || !SC || !slots || (lanes_per_node != 1 && lanes_per_node != 2))
+        return fail(BNM_E_ARG, "bnm_sparse_layout: bad arguments");
+    const int N = nodes, LS = lanes_per_node;
+    *slots = 0;
+    if (N > 512 / LS) return BNM_OK;
+    const size_t M = (size_t)LS * N;
+    std::vector<unsigned> d;
+    int P = 0;
+    if (fp64) {
+        std::vector<double> w;
+        P = build_sparse_sc(SC, N, LS, w, d);
+        if (P <= 0 || P > kSparseRW64) return BNM_OK;
+        if (W) std::copy(w.begin(), w.begin() + (size_t)P * M, W);
+    } else {
+        std::vector<float> f((size_t)N * N), w;
+        for (size_t i = 0; i < f.size(); ++i) f[i] = (float)SC[i];
+        P = build_sparse_sc(f.data(), N, LS, w, d);
+        if (P <= 0 || P > kSparseRW + kSparseLL) return BNM_OK;
+        if (W) for (size_t i = 0; i < (size_t)P * M; ++i) W[i] = (double)w[i];
+    }
+    if (D) std::copy(d.begin(), d.begin() + (size_t)P / 4 * M, D);
+    *slots = P;
+    return BNM_OK;
+}
+
 int bnm_make_noise_chunked(unsigned seed, long long count, float* out, int threads) {
     if (!out || count < 0) return fail(BNM_E_ARG, "bad noise buffer");
     constexpr long long kChunk = 1LL << 20;

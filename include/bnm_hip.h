@@ -153,6 +153,21 @@ int bnm_make_noise(unsigned seed, long long count, float* out);
  * whose driver draws one sequential stream (run_CMAES.cpp:640-657). */
 int bnm_make_noise_chunked(unsigned seed, long long count, float* out, int threads);
 
+/* The nonzero lists the sparse integration kernel builds from an N x N SC
+ * (row-major, FP64) at bnm_ctx_set_sc, for inspection (host only, no GPU
+ * work).  lanes_per_node 1 or 2 (2: each lane takes half of the row's
+ * k-range, ceil(N/2) nodes); fp64 selects the FP64 layout (8-byte steps)
+ * over the FP32 one (4-byte steps, weights rounded to FP32).  *slots gets P,
+ * the slots per lane (0 when the kernel does not apply: N > 512 / lanes,
+ * no nonzero, or more nonzeros than it holds).  When P > 0 and W / D are
+ * non-null they receive, for p < P and lane column l = sl * N + j (slice sl
+ * of row j): W[p * lanes_per_node * N + l] the slot's weight and, in byte
+ * p % 4 of D[(p / 4) * lanes_per_node * N + l], its step in bytes from the
+ * previous slot's S_E (the first from the slice's first node).  No
+ * counterpart in the reference. */
+int bnm_sparse_layout(int nodes, const double* SC, int lanes_per_node, int fp64, int* slots, double* W,
+                      unsigned* D);
+
 /* Replaces run_simulations (GPU) of bnm.cu:594-617.  Host arrays in, host
  * arrays out; outputs that are NULL are not produced.
  *   corr[B]            goodness-of-fit cost 1 - r(FC) + KS(FCD) [+ |dmean FC|]
