@@ -190,6 +190,64 @@ struct NodeStrict {
     }
 };
 
+// NodeStrict's update of one pool, with the pool chosen per lane at run time
+// (excitatory or inhibitory constants held in registers), so that the two
+// lanes of a node run one instruction stream, one pool each.  The same
+// expressions as NodeStrict::pool<P>: for the inhibitory pool the coupling
+// and wIE terms have zero coefficients (c and S finite), which adds exact
+// zeros, so both give the same bits.
+struct PoolStrict {
+    float k0h = 0, k0l = 0, kSEh = 0, kSEl = 0, kSIh = 0, kSIl = 0;   // pre = k0 + kSE S_E - kSI S_I
+    float ch = 0, cl = 0, GJp = 0;                                     // coupling (excitatory only)
+    float aa = 0, bb = 0, ndlh = 0, ndll = 0, invd = 0, d12 = 0;
+    float g = 0, gl = 0, gm = 0, gml = 0, it = 0, itl = 0;
+    __device__ __forceinline__ void init(int pop, double G, double wEE, double wEI, double wIE) {
+        const bool e = pop == 0;
+        const double k0 = e ? C64::wE_I0 : C64::wI_I0, a = e ? C64::a_E : C64::a_I, b = e ? C64::b_E : C64::b_I;
+        const double d = e ? C64::d_E : C64::d_I;
+        const double g64 = C64::dt * (e ? C64::gamma_E : C64::gamma_I), it64 = C64::dt / (e ? C64::tau_E : C64::tau_I);
+        NodeStrict::split(k0, k0h, k0l);
+        NodeStrict::split(e ? wEE : wEI, kSEh, kSEl);
+        NodeStrict::split(e ? wIE : 1.0, kSIh, kSIl);
+        NodeStrict::split(e ? a * (G * C64::J_NMDA) : 0.0, ch, cl);
+        GJp = e ? (float)(G * C64::J_NMDA) : 0.0f;
+        aa = (float)a;
+        bb = (float)b;
+        NodeStrict::split(-d * kLog2e, ndlh, ndll);
+        invd = (float)(1.0 / d);
+        d12 = (float)(d / 12.0);
+        NodeStrict::split(g64, g, gl);
+        gm = e ? g : 0.0f;
+        gml = e ? gl : 0.0f;
+        NodeStrict::split(it64, it, itl);
+    }
+    __device__ __forceinline__ void set_wIE(double wIE) { NodeStrict::split(wIE, kSIh, kSIl); }
+
+    // one substep of this lane's pool from the previous S_E, S_I; returns r
+    // and the pool's input current I
+    __device__ __forceinline__ void step(float c, float xi, float SEv, float SIv, float& S, float& Sl, float& r,
+                                         float& I) const {
+        constexpr double nzs64 = C64::sigma * C64::sqrt_dt;
+        constexpr float nzs = (float)nzs64, nzsl = (float)(nzs64 - (double)nzs);
+        const float pre = fmaf(-kSIh, SIv, fmaf(kSEh, SEv, k0h));
+        const float prel = fmaf(-kSIl, SIv, fmaf(kSEl, SEv, k0l));
+        const float X0s = fmaf(aa, prel, fmaf(aa, pre, -bb));
+        const float x = fmaf(ch, c, fmaf(cl, c, X0s));
+        const float y = fmaf(ndlh, x, ndll * x);
+        const float direct = x * __builtin_amdgcn_rcpf(1.0f - __builtin_amdgcn_exp2f(y));
+        const float series = invd + x * (0.5f + d12 * x);
+        r = fabsf(x) < 0.1f ? series : direct;
+        const float Ac = fmaf(-gm, S, g), Acl = fmaf(-gml, S, gl);
+        const float D = fmaf(xi, nzsl, fmaf(-itl, S, fmaf(-it, S, xi * nzs))) + Sl;
+        const float u = fmaf(r, Ac, fmaf(r, Acl, D));
+        const float t = S + u;
+        const float tc = __builtin_amdgcn_fmed3f(t, 0.0f, 1.0f);
+        Sl = tc == t ? u - (t - S) : 0.0f;
+        S = tc;
+        I = fmaf(GJp, c, pre) + prel;
+    }
+};
+
 // Balloon-Windkessel state; one Euler step of 1 ms (bnm.cpp:205-211), update
 // order: x from old f, f from new x, nu and q from old f and (q) new nu.
 template <class R> struct BW;

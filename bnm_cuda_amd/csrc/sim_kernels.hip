@@ -1462,7 +1462,11 @@ __global__ __launch_bounds__(MAXT) void k_sim_hyb(const float* __restrict__ SCc,
 //
 // STRICT (FP32): the node update is NodeStrict (hi / lo coefficients, S as
 // S + Sl, FIC in FP64, extended-output sums in place in a.ext); otherwise
-// Node<R> (FP64: the reference's expression order).
+// Node<R> (FP64: the reference's expression order).  With two lanes per node
+// the strict update is split by pool (PoolStrict: the lower lane of the pair
+// steps the excitatory pool, the upper the inhibitory, both from the previous
+// S_E and S_I exchanged by one permlane32 swap), which halves the update's
+// VALU work per lane and gives NodeStrict's bits.
 //
 // A zero weight adds +0 to the dense dot as well (S_E is finite and >= 0),
 // so this kernel and the dense ones differ only in the grouping of the sum:
@@ -1477,6 +1481,10 @@ __global__ __launch_bounds__(MAXT) void k_sim_sparse(const R* __restrict__ Wsp, 
     static_assert(std::is_same<R, float>::value || (!STRICT && LLMAX == 0),
                   "FP64: the reference-order update, all weights in registers");
     constexpr unsigned ES = sizeof(R);
+    // strict with two lanes per node: the lanes split the pools (lane 0 of
+    // the pair excitatory, lane 1 inhibitory; PoolStrict), the partner's S
+    // from a permlane32 swap
+    constexpr bool SPLIT = STRICT && LS == 2;
     static_assert(LS == 1 || LS == 2, "one or two lanes per node");
     constexpr int DW = (RW + LLMAX) / 4;
     constexpr int GN = 64 / LS;   // nodes per wave
@@ -1523,17 +1531,23 @@ __global__ __launch_bounds__(MAXT) void k_sim_sparse(const R* __restrict__ Wsp, 
     float SEl = NodeStrict::kS0l, SIl = NodeStrict::kS0l;
     double wIEd = 0.0, deltad = a.init_delta;
     // (the strict form sums its extended outputs in place in a.ext, once per
-    // TR sample, which keeps six FP64 sums out of the VGPRs)
+    // TR sample, which keeps six FP64 sums out of the VGPRs; split, each lane
+    // owns its pool's three)
     double* extg = a.ext + (size_t)sim * 6 * N + j;
+    const bool ext_owner = SPLIT ? comp : writer;
+    const int e0 = SPLIT ? 3 * sl : 0, e1 = SPLIT ? e0 + 3 : 6;
+    PoolStrict ps;
+    float S_own = 0.001f, Sl_own = NodeStrict::kS0l, r_own = 0.0f, I_own = 0.0f;   // split: this lane's pool
     if constexpr (STRICT) {
         ns.set_G(a.G[sim]);
         if (comp) {
             const size_t o = (size_t)sim * N + j;
             wIEd = a.wIE_in[o];
-            ns.set_w(a.wEE[o], a.wEI[o], wIEd);
+            if constexpr (SPLIT) ps.init(sl, a.G[sim], a.wEE[o], a.wEI[o], wIEd);
+            else ns.set_w(a.wEE[o], a.wEI[o], wIEd);
         }
-        if (writer)
-            for (int i = 0; i < 6; ++i) extg[(size_t)i * N] = 0.0;
+        if (ext_owner)
+            for (int i = e0; i < e1; ++i) extg[(size_t)i * N] = 0.0;
     }
     const R gp = gain<R>(a.G[sim]);
     int ts = 0, vol = 0, trial = 0, buf = 0;
@@ -1604,7 +1618,16 @@ __global__ __launch_bounds__(MAXT) void k_sim_sparse(const R* __restrict__ Wsp, 
             // compiler sinks the products into the node-lane branch below and
             // keeps every gathered value live until then
             asm volatile("" : : "v"(c));
-            if (comp) {
+            if constexpr (SPLIT) {
+                // S_E is the lower lane's value of the swap pair, S_I the upper's
+                const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(S_own), __float_as_uint(S_own), false,
+                                                                 false);
+                if (comp) {
+                    ps.step(c, sl ? nz.y : nz.x, __uint_as_float(sw[0]), __uint_as_float(sw[1]), S_own, Sl_own, r_own,
+                            I_own);
+                    if (sl == 0) Sb[(buf ^ 1) * ldS + j] = S_own;
+                }
+            } else if (comp) {
                 if constexpr (STRICT)
                     ns.step(c, nz.x, nz.y, L.SE, SEl, L.SI, SIl, L.rE, L.rI, &L.IE, &L.II);
                 else
@@ -1614,9 +1637,19 @@ __global__ __launch_bounds__(MAXT) void k_sim_sparse(const R* __restrict__ Wsp, 
             __syncthreads();
             buf ^= 1;
         }
-        if (comp) L.bw.step(L.SE);
+        if (comp) L.bw.step(SPLIT ? S_own : L.SE);
         if (ts % a.bold_tr == 0) {
-            if (writer) {
+            if constexpr (SPLIT) {
+                if (writer) {
+                    float* bold = reinterpret_cast<float*>(a.bold);
+                    bold[((size_t)sim * a.output_ts + vol) * N + j] = L.bw.bold(a.bw_k1, a.bw_k2, a.bw_k3);
+                }
+                if (comp && vol >= a.n_vols_remove) {
+                    extg[(size_t)e0 * N] += (double)S_own + (double)Sl_own;
+                    extg[(size_t)(e0 + 1) * N] += (double)I_own;
+                    extg[(size_t)(e0 + 2) * N] += (double)r_own;
+                }
+            } else if (writer) {
                 if constexpr (STRICT) {
                     float* bold = reinterpret_cast<float*>(a.bold);
                     bold[((size_t)sim * a.output_ts + vol) * N + j] = L.bw.bold(a.bw_k1, a.bw_k2, a.bw_k3);
@@ -1636,11 +1669,17 @@ __global__ __launch_bounds__(MAXT) void k_sim_sparse(const R* __restrict__ Wsp, 
         }
         ++ts;
         if (adjusting) {
-            if (comp && ts >= a.i_start && ts <= a.i_end) L.mIE += (double)L.IE;
+            if (comp && ts >= a.i_start && ts <= a.i_end) L.mIE += (double)(SPLIT ? I_own : L.IE);
             if (ts == a.i_end) {
                 bool out = false;
                 if (comp) {
-                    if constexpr (STRICT) {
+                    if constexpr (SPLIT) {
+                        // the excitatory lane decides; the inhibitory pool has no wIE
+                        if (sl == 0) {
+                            out = fic_decide<double>(L.mIE, dur, trial, a.max_fic_trials, wIEd, deltad);
+                            ps.set_wIE(wIEd);
+                        }
+                    } else if constexpr (STRICT) {
                         out = fic_decide<double>(L.mIE, dur, trial, a.max_fic_trials, wIEd, deltad);
                         ns.set_wIE(wIEd);
                     } else {
@@ -1652,10 +1691,12 @@ __global__ __launch_bounds__(MAXT) void k_sim_sparse(const R* __restrict__ Wsp, 
                         if (comp) {
                             L.reset();
                             SEl = SIl = NodeStrict::kS0l;
+                            S_own = 0.001f;
+                            Sl_own = NodeStrict::kS0l;
                             if (sl == 0) Sb[buf * ldS + j] = R(0.001);
                         }
-                        if (STRICT && writer)
-                            for (int i = 0; i < 6; ++i) extg[(size_t)i * N] = 0.0;
+                        if (STRICT && ext_owner)
+                            for (int i = e0; i < e1; ++i) extg[(size_t)i * N] = 0.0;
                         ts = 0; vol = 0; ++trial;
                         __syncthreads();
                     } else {
@@ -1668,10 +1709,10 @@ __global__ __launch_bounds__(MAXT) void k_sim_sparse(const R* __restrict__ Wsp, 
         }
     }
     if constexpr (STRICT) {
-        if (writer) {
-            a.wIE_out[(size_t)sim * N + j] = wIEd;
+        if (writer) a.wIE_out[(size_t)sim * N + j] = wIEd;
+        if (ext_owner) {
             const int n = vol - a.n_vols_remove;
-            for (int i = 0; i < 6; ++i) extg[(size_t)i * N] = n > 0 ? extg[(size_t)i * N] / n : 0.0;
+            for (int i = e0; i < e1; ++i) extg[(size_t)i * N] = n > 0 ? extg[(size_t)i * N] / n : 0.0;
         }
         if (threadIdx.x == 0) {
             a.fic_failed[sim] = failed ? 1 : 0;

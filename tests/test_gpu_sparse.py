@@ -92,6 +92,11 @@ def _check_summary(oracle_mod, p, g, r, bold_rel=1e-5):
     np.testing.assert_array_equal(g["fic_trials"], r["fic_trials"])
     assert d_re <= 0.05 and d_fc <= 1e-3 and d_ks <= 0.01
     assert rel <= bold_rel
+    # all six extended outputs (S_E, I_E, r_E, S_I, I_I, r_I means; with two
+    # lanes per node the inhibitory lane writes the last three)
+    for q in range(6):
+        scale = np.abs(r["ext"][:, q]).max()
+        assert np.abs(g["ext"][:, q] - r["ext"][:, q]).max() <= 1e-4 * scale, q
 
 
 @pytest.mark.parametrize("N,seed,dens,ls,want", [(400, 13, 0.40, "1", "LS=1,P=200,regs=112,lds=88"),
