@@ -188,6 +188,21 @@ __device__ __forceinline__ double swap_f64(double v, bool s32) {
 }
 template <> __device__ __forceinline__ double lane_sum32<double>(double v) { return swap_f64(v, true); }
 template <> __device__ __forceinline__ double lane_sum16<double>(double v) { return swap_f64(v, false); }
+// both values of a permlane32 swap pair in every lane: the lower lane's
+// (lanes 0-31) and the upper lane's (32-63)
+__device__ __forceinline__ void pair32(float v, float& lower, float& upper) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    lower = __uint_as_float(r[0]);
+    upper = __uint_as_float(r[1]);
+}
+__device__ __forceinline__ void pair32(double v, double& lower, double& upper) {
+    const unsigned long long u = __double_as_longlong(v);
+    const unsigned lo = (unsigned)u, hi = (unsigned)(u >> 32);
+    const auto rl = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    const auto rh = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    lower = __longlong_as_double(((unsigned long long)rh[0] << 32) | rl[0]);
+    upper = __longlong_as_double(((unsigned long long)rh[1] << 32) | rl[1]);
+}
 
 template <class R, int KC, int LS, int MAXT>
 __global__ __launch_bounds__(MAXT) void k_sim_split(const R* __restrict__ SC, SimArgs a, int NW,
@@ -1463,10 +1478,11 @@ __global__ __launch_bounds__(MAXT) void k_sim_hyb(const float* __restrict__ SCc,
 // STRICT (FP32): the node update is NodeStrict (hi / lo coefficients, S as
 // S + Sl, FIC in FP64, extended-output sums in place in a.ext); otherwise
 // Node<R> (FP64: the reference's expression order).  With two lanes per node
-// the strict update is split by pool (PoolStrict: the lower lane of the pair
-// steps the excitatory pool, the upper the inhibitory, both from the previous
-// S_E and S_I exchanged by one permlane32 swap), which halves the update's
-// VALU work per lane and gives NodeStrict's bits.
+// the strict and the FP64 update are split by pool (the lower lane of the
+// pair steps the excitatory pool, the upper the inhibitory, both from the
+// previous S_E and S_I exchanged by a permlane32 swap; PoolStrict gives
+// NodeStrict's bits, ei_step_f64 Node<double>'s), which halves the update's
+// VALU work per lane.
 //
 // A zero weight adds +0 to the dense dot as well (S_E is finite and >= 0),
 // so this kernel and the dense ones differ only in the grouping of the sum:
@@ -1481,10 +1497,13 @@ __global__ __launch_bounds__(MAXT) void k_sim_sparse(const R* __restrict__ Wsp, 
     static_assert(std::is_same<R, float>::value || (!STRICT && LLMAX == 0),
                   "FP64: the reference-order update, all weights in registers");
     constexpr unsigned ES = sizeof(R);
-    // strict with two lanes per node: the lanes split the pools (lane 0 of
-    // the pair excitatory, lane 1 inhibitory; PoolStrict), the partner's S
-    // from a permlane32 swap
-    constexpr bool SPLIT = STRICT && LS == 2;
+    // two lanes per node split the node update by pool (the lower lane of the
+    // pair excitatory, the upper inhibitory), the partner's S from a
+    // permlane32 swap: strict FP32 with PoolStrict, FP64 with ei_step_f64
+    // (up to 48 register slots: its per-lane pool constants take 20 VGPRs,
+    // which the 64-slot build does not have without spills)
+    constexpr bool F64 = std::is_same<R, double>::value;
+    constexpr bool SPLIT = LS == 2 && (STRICT || (F64 && RW <= 48));
     static_assert(LS == 1 || LS == 2, "one or two lanes per node");
     constexpr int DW = (RW + LLMAX) / 4;
     constexpr int GN = 64 / LS;   // nodes per wave
@@ -1537,7 +1556,14 @@ __global__ __launch_bounds__(MAXT) void k_sim_sparse(const R* __restrict__ Wsp, 
     const bool ext_owner = SPLIT ? comp : writer;
     const int e0 = SPLIT ? 3 * sl : 0, e1 = SPLIT ? e0 + 3 : 6;
     PoolStrict ps;
-    float S_own = 0.001f, Sl_own = NodeStrict::kS0l, r_own = 0.0f, I_own = 0.0f;   // split: this lane's pool
+    R S_own = R(0.001), r_own = 0, I_own = 0;   // split: this lane's pool (+ Sl_own, strict)
+    float Sl_own = NodeStrict::kS0l;
+    // FP64 split: the pool's constants of ei_step_f64 (the inhibitory pool
+    // has no coupling, kSI = 1 and m = 0)
+    const double pGl = sl ? 0.0 : a.G[sim], pk0 = sl ? C64::wI_I0 : C64::wE_I0;
+    const double pa = sl ? C64::a_I : C64::a_E, pb = sl ? C64::b_I : C64::b_E, pd = sl ? C64::d_I : C64::d_E;
+    const double pg = sl ? C64::gamma_I : C64::gamma_E, pm = sl ? 0.0 : 1.0, ptau = sl ? C64::tau_I : C64::tau_E;
+    const double pkSE = sl ? (double)L.wEI : (double)L.wEE;
     if constexpr (STRICT) {
         ns.set_G(a.G[sim]);
         if (comp) {
@@ -1619,12 +1645,14 @@ __global__ __launch_bounds__(MAXT) void k_sim_sparse(const R* __restrict__ Wsp, 
             // keeps every gathered value live until then
             asm volatile("" : : "v"(c));
             if constexpr (SPLIT) {
-                // S_E is the lower lane's value of the swap pair, S_I the upper's
-                const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(S_own), __float_as_uint(S_own), false,
-                                                                 false);
+                R SEv, SIv;   // S_E from the lower lane of the pair, S_I from the upper
+                pair32(S_own, SEv, SIv);
                 if (comp) {
-                    ps.step(c, sl ? nz.y : nz.x, __uint_as_float(sw[0]), __uint_as_float(sw[1]), S_own, Sl_own, r_own,
-                            I_own);
+                    if constexpr (STRICT)
+                        ps.step(c, sl ? nz.y : nz.x, SEv, SIv, S_own, Sl_own, r_own, I_own);
+                    else
+                        ei_step_f64(c, pGl, pk0, pkSE, sl ? 1.0 : L.wIE, pa, pb, pd, pg, pm, ptau,
+                                    (double)(sl ? nz.y : nz.x), SEv, SIv, S_own, I_own, r_own);
                     if (sl == 0) Sb[(buf ^ 1) * ldS + j] = S_own;
                 }
             } else if (comp) {
@@ -1641,13 +1669,19 @@ __global__ __launch_bounds__(MAXT) void k_sim_sparse(const R* __restrict__ Wsp, 
         if (ts % a.bold_tr == 0) {
             if constexpr (SPLIT) {
                 if (writer) {
-                    float* bold = reinterpret_cast<float*>(a.bold);
+                    R* bold = reinterpret_cast<R*>(a.bold);
                     bold[((size_t)sim * a.output_ts + vol) * N + j] = L.bw.bold(a.bw_k1, a.bw_k2, a.bw_k3);
                 }
                 if (comp && vol >= a.n_vols_remove) {
-                    extg[(size_t)e0 * N] += (double)S_own + (double)Sl_own;
-                    extg[(size_t)(e0 + 1) * N] += (double)I_own;
-                    extg[(size_t)(e0 + 2) * N] += (double)r_own;
+                    if constexpr (STRICT) {
+                        extg[(size_t)e0 * N] += (double)S_own + (double)Sl_own;
+                        extg[(size_t)(e0 + 1) * N] += (double)I_own;
+                        extg[(size_t)(e0 + 2) * N] += (double)r_own;
+                    } else {
+                        L.ext[0] += (double)S_own;
+                        L.ext[1] += (double)I_own;
+                        L.ext[2] += (double)r_own;
+                    }
                 }
             } else if (writer) {
                 if constexpr (STRICT) {
@@ -1676,8 +1710,12 @@ __global__ __launch_bounds__(MAXT) void k_sim_sparse(const R* __restrict__ Wsp, 
                     if constexpr (SPLIT) {
                         // the excitatory lane decides; the inhibitory pool has no wIE
                         if (sl == 0) {
-                            out = fic_decide<double>(L.mIE, dur, trial, a.max_fic_trials, wIEd, deltad);
-                            ps.set_wIE(wIEd);
+                            if constexpr (STRICT) {
+                                out = fic_decide<double>(L.mIE, dur, trial, a.max_fic_trials, wIEd, deltad);
+                                ps.set_wIE(wIEd);
+                            } else {
+                                out = fic_decide<R>(L.mIE, dur, trial, a.max_fic_trials, L.wIE, L.delta);
+                            }
                         }
                     } else if constexpr (STRICT) {
                         out = fic_decide<double>(L.mIE, dur, trial, a.max_fic_trials, wIEd, deltad);
@@ -1691,7 +1729,7 @@ __global__ __launch_bounds__(MAXT) void k_sim_sparse(const R* __restrict__ Wsp, 
                         if (comp) {
                             L.reset();
                             SEl = SIl = NodeStrict::kS0l;
-                            S_own = 0.001f;
+                            S_own = R(0.001);
                             Sl_own = NodeStrict::kS0l;
                             if (sl == 0) Sb[buf * ldS + j] = R(0.001);
                         }
@@ -1713,6 +1751,17 @@ __global__ __launch_bounds__(MAXT) void k_sim_sparse(const R* __restrict__ Wsp, 
         if (ext_owner) {
             const int n = vol - a.n_vols_remove;
             for (int i = e0; i < e1; ++i) extg[(size_t)i * N] = n > 0 ? extg[(size_t)i * N] / n : 0.0;
+        }
+        if (threadIdx.x == 0) {
+            a.fic_failed[sim] = failed ? 1 : 0;
+            a.fic_trials[sim] = trial;
+        }
+    } else if constexpr (SPLIT) {
+        // finish() for a lane that holds one pool's three sums
+        if (comp) {
+            if (sl == 0) a.wIE_out[(size_t)sim * N + j] = (double)L.wIE;
+            const int n = vol - a.n_vols_remove;
+            for (int i = 0; i < 3; ++i) a.ext[((size_t)sim * 6 + e0 + i) * N + j] = n > 0 ? L.ext[i] / n : 0.0;
         }
         if (threadIdx.x == 0) {
             a.fic_failed[sim] = failed ? 1 : 0;

@@ -14,8 +14,9 @@ The default sparse form has the strict FP32 node update (hi / lo
 coefficients, compensated S, FIC in FP64): per-TR BOLD within 1e-6 relative
 over the first 60 s at N = 200 and 400.
 
-The FP64 form (N <= 256 with two lanes per node, all slots in registers)
-meets the FP64 bounds against the oracle.
+The FP64 form (N <= 256 with two lanes per node, all slots in registers,
+the node update split by pool over the two lanes) meets the FP64 bounds
+against the oracle.
 
 Edge cases: a block-structured SC with isolated nodes (empty rows) and gaps
 of more than 63 nodes between a row's nonzeros (zero-weight step slots); an
@@ -233,9 +234,9 @@ def test_sparse_strict_per_tr_bold_within_1e6_over_60s(oracle_mod, monkeypatch):
 
 def test_sparse_f64_matches_oracle(oracle_mod, monkeypatch):
     """FP64 sparse kernel (two lanes per node, FP64 weights and S_E, the
-    reference-order node update): against the oracle at the FP64 bounds of
-    the uneven-graph test (per-TR BOLD 1e-6 relative, cost and FC 1e-9),
-    same FIC trials."""
+    reference-order node update split by pool over the two lanes): against
+    the oracle at the FP64 bounds of the uneven-graph test (per-TR BOLD 1e-6
+    relative, cost and FC 1e-9), same FIC trials and wIE."""
     print("case: sparse f64 N=200 LS=2", flush=True)
     p = _pop(200, W.synthetic_sc(200, 0.33, 12), 3, 10_000, 12)
     g, name = _run(p, monkeypatch, {"BNM_SPARSE": "2"}, precision=bnm.PREC_F64)
@@ -249,3 +250,4 @@ def test_sparse_f64_matches_oracle(oracle_mod, monkeypatch):
     np.testing.assert_allclose(g["corr"], r["corr"], rtol=0, atol=1e-9)
     np.testing.assert_allclose(g["fc"], r["fc"], rtol=0, atol=1e-9)
     np.testing.assert_allclose(g["ext"], r["ext"], rtol=1e-7, atol=1e-9)
+    np.testing.assert_allclose(g["wIE_fic"], r["wIE_fic"], rtol=0, atol=1e-12)
