@@ -75,6 +75,24 @@ template <> struct Node<double> {
     }
 };
 
+// FP64 update of one pool with the pool's constants as arguments (the FP64
+// E/I kernels: lanes that each step one pool).  Excitatory: k0 = wE_I0,
+// kSE = wEE, Gl = G, kSI = wIE, m = 1; inhibitory: k0 = wI_I0, kSE = wEI,
+// Gl = 0, kSI = 1, m = 0, whose extra terms are exact ((c*0)*J = +0 for
+// c >= 0, 1*S_I, (1 - 0*S)*gamma = gamma), so each pool evaluates
+// Node<double>'s expression for it.
+__device__ __forceinline__ void ei_step_f64(double c, double Gl, double k0, double kSE, double kSI, double aa,
+                                            double bb, double dd, double g, double m, double tau, double xi,
+                                            double SEv, double SIv, double& S, double& I, double& r) {
+#pragma clang fp contract(off)
+    I = k0 + kSE * SEv + c * Gl * C64::J_NMDA - kSI * SIv;
+    const double x = aa * I - bb;
+    const double e = exp(-dd * x);
+    r = x / (1 - e);
+    const double dS = xi * C64::sigma * C64::sqrt_dt + C64::dt * ((1 - m * S) * g * r - (S / tau));
+    S = fmax(0.0, fmin(S + dS, 1.0));
+}
+
 // x / (1 - exp(-d x)), FP32.  Near x = 0 the quotient is 1/d + x/2 + d x^2/12
 // (series of the removable singularity); the series is used for |x| < 0.1,
 // where the direct form would lose digits to cancellation.

@@ -428,19 +428,6 @@ __device__ __forceinline__ R lds_load(unsigned addr) {
     return *(const __attribute__((address_space(3))) R*)(size_t)addr;
 }
 
-// FP64 node update of one pool (the FP64 E/I kernel's step, below).
-__device__ __forceinline__ void ei_step_f64(double c, double Gl, double k0, double kSE, double kSI, double aa,
-                                            double bb, double dd, double g, double m, double tau, double xi,
-                                            double SEv, double SIv, double& S, double& I, double& r) {
-#pragma clang fp contract(off)
-    I = k0 + kSE * SEv + c * Gl * C64::J_NMDA - kSI * SIv;
-    const double x = aa * I - bb;
-    const double e = exp(-dd * x);
-    r = x / (1 - e);
-    const double dS = xi * C64::sigma * C64::sqrt_dt + C64::dt * ((1 - m * S) * g * r - (S / tau));
-    S = fmax(0.0, fmin(S + dS, 1.0));
-}
-
 // FORM: 0 the update as written above; 1 the same with the DPP row broadcast;
 // 2 / 3 the short-chain update: everything that does not depend on the
 // coupling c is formed before it arrives, so that after the reduction
