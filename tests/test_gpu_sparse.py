@@ -232,19 +232,21 @@ def test_sparse_strict_per_tr_bold_within_1e6_over_60s(oracle_mod, monkeypatch):
     assert mx <= 1e-6
 
 
-def test_sparse_f64_matches_oracle(oracle_mod, monkeypatch):
+@pytest.mark.parametrize("dens,seed,regs", [(0.33, 12, "regs=48"), (0.40, 5, "regs=64")])
+def test_sparse_f64_matches_oracle(oracle_mod, monkeypatch, dens, seed, regs):
     """FP64 sparse kernel (two lanes per node, FP64 weights and S_E, the
-    reference-order node update split by pool over the two lanes): against
+    reference-order node update; up to 48 register slots split by pool over
+    the two lanes, the 64-slot build with both pools in each lane): against
     the oracle at the FP64 bounds of the uneven-graph test (per-TR BOLD 1e-6
     relative, cost and FC 1e-9), same FIC trials and wIE."""
-    print("case: sparse f64 N=200 LS=2", flush=True)
-    p = _pop(200, W.synthetic_sc(200, 0.33, 12), 3, 10_000, 12)
+    print(f"case: sparse f64 N=200 density {dens} LS=2 {regs}", flush=True)
+    p = _pop(200, W.synthetic_sc(200, dens, seed), 3, 10_000, seed)
     g, name = _run(p, monkeypatch, {"BNM_SPARSE": "2"}, precision=bnm.PREC_F64)
     r = _oracle(oracle_mod, p)
     rel = max(np.abs(g["bold"][b] - r["bold"][b]).max() / np.abs(r["bold"][b]).max() for b in range(p["B"]))
     print(name, f"| per-TR BOLD rel {rel:.3g}, cost {np.abs(g['corr'] - r['corr']).max():.3g}, "
           f"FC {np.abs(g['fc'] - r['fc']).max():.3g}", flush=True)
-    assert name.startswith("sparse_f64(LS=2")
+    assert name.startswith("sparse_f64(LS=2") and regs in name
     np.testing.assert_array_equal(g["fic_trials"], r["fic_trials"])
     assert rel <= 1e-6
     np.testing.assert_allclose(g["corr"], r["corr"], rtol=0, atol=1e-9)
